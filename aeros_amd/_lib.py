@@ -1,0 +1,94 @@
+"""ctypes binding of libaer_sht.so (the C ABI in include/aer_sht.h).
+
+The library is built in-tree by aeros_amd.build (or __graft_entry__.build()).
+There is no fallback: if the shared object is missing or cannot be loaded,
+every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libaer_sht.so")
+
+AER_OK, AER_EINVAL, AER_ENOMEM, AER_EHIP, AER_ENODEV, AER_ESTATE = range(6)
+
+AER_SHT_ORTHONORMAL = 0x1
+AER_SHT_NO_CS_PHASE = 0x2
+AER_SHT_PHI_CONTIGUOUS = 0x10
+AER_SHT_GAUSS = 0x20
+AER_SHT_QUICK_INIT = 0x40
+AER_SHT_LOAD_SAVE_CFG = 0x80
+
+AER_PREC_FP64, AER_PREC_FP32, AER_PREC_SPLIT = 0, 1, 2
+
+# Every symbol include/aer_sht.h declares, with (restype, argtypes).
+_c_int, _c_long, _c_double, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ShtInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("lmax", "mmax", "mres", "nlm", "nlat", "nphi", "nspat", "nlat_2")]
+
+
+SIGNATURES = {
+    "aer_strerror": (ctypes.c_char_p, [_c_int]),
+    "aer_last_error": (ctypes.c_char_p, []),
+    "aer_sht_grid_size": (_c_int, [_c_int, _ip, _ip]),
+    "aer_sht_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "aer_sht_set_grid": (_c_int, [_vp, _c_int, _c_double, _c_int, _c_int]),
+    "aer_sht_get_info": (_c_int, [_vp, ctypes.POINTER(ShtInfo)]),
+    "aer_sht_get_lm": (_c_int, [_vp, _ip, _ip]),
+    "aer_sht_get_cos_theta": (_c_int, [_vp, _dp]),
+    "aer_sht_gauss_wts": (_c_int, [_vp, _dp]),
+    "aer_sht_destroy": (_c_int, [_vp]),
+    "aer_sht_set_precision": (_c_int, [_vp, _c_int]),
+    "aer_sht_set_stream": (_c_int, [_vp, _vp]),
+    "aer_sht_sync": (_c_int, [_vp]),
+    "aer_spat_to_SH": (_c_int, [_vp, _vp, _vp]),
+    "aer_SH_to_spat": (_c_int, [_vp, _vp, _vp]),
+    "aer_spat_to_SHsphtor": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "aer_SHsphtor_to_spat": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "aer_SHsph_to_spat": (_c_int, [_vp, _vp, _vp, _vp]),
+    "aer_synth_d": (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long]),
+    "aer_anal_d": (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long]),
+    "aer_synth_vec_d": (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _vp, _c_long]),
+    "aer_anal_vec_d": (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _vp, _c_long]),
+    "aer_synth_grad_d": (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _vp, _c_long]),
+    "aer_uv_from_vordiv_d": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
+    "aer_vordiv_from_uv_d": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp]),
+    "aer_gradient_d": (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    "aer_tendency_grid_transforms_d": (_c_int, [_vp, _c_int] + [_vp] * 14),
+    "aer_tendency_spectral_d": (_c_int, [_vp, _c_int] + [_vp] * 9),
+}
+
+
+class AerError(RuntimeError):
+    def __init__(self, status, detail):
+        self.status = status
+        super().__init__(f"aer status {status}: {detail}")
+
+
+_lib = None
+
+
+def lib():
+    """Load libaer_sht.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != AER_OK:
+        raise AerError(rc, lib().aer_last_error().decode())
+    return rc

@@ -1,0 +1,780 @@
+// aer_sht.hip -- host side of the gfx950 spherical-harmonic transform library:
+// the C ABI declared in include/aer_sht.h, FP64 geometry and Legendre tables,
+// FFT plans, and the job builder that turns each API call into one launch per
+// stage (see sht_device.h for the kernels).
+#include "sht_device.h"
+#include "../../include/aer_sht.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kREarth = 6.371e6;  // reference src/aeros_defs.f90:84
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(AER_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n)
+    {
+        if (n <= bytes) return AER_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) return fail(AER_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        bytes = n;
+        return AER_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace
+
+struct aer_sht {
+    int lmax = 0, mmax = 0, mres = 1, norm = 0, layout = 0;
+    int nlat = 0, nlon = 0, nlm = 0, nh = 0, nhp = 0, npt = 0, pb = 1, fb = 8;
+    double eps = 0.0;
+    bool grid_set = false;
+    int prec = AER_PREC_FP64;
+    hipStream_t stream = nullptr;
+    std::vector<double> ct, w;             // nodes (cos colatitude) N->S, weights
+    std::vector<int> li, mi;
+    std::vector<int> radix, twoff;
+    // device data
+    DevBuf tab64[3], tab32[3], lfac, wpair, tw64, tw32, four, stage_s, stage_g;
+};
+
+namespace {
+
+// ------------------------------------------------------------------ host math
+
+// 2/3/5-smooth and even (reference src/aeros_spectral.f90:440-458).
+bool smooth_even(int n)
+{
+    if (n <= 0 || n % 2) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    while (n % 5 == 0) n /= 5;
+    return n == 1;
+}
+
+// Gauss-Legendre nodes (descending) and weights: Newton on P_n(x) from the
+// three-term recurrence; 1 - x^2 formed as (1-x)(1+x), exact to rounding for the
+// represented node, so the polar weights keep full accuracy. The southern half
+// is the exact mirror of the northern half.
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const int nh = (n + 1) / 2;
+    for (int i = 0; i < nh; ++i) {
+        double z = std::cos(kPi * (i + 0.75) / (n + 0.5));
+        double omz2 = 1.0, dp = 1.0;
+        for (int it = 0; it <= 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            omz2 = (1.0 - z) * (1.0 + z);
+            dp = n * (p0 - z * p1) / omz2;
+            if (it == 100) break;
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) <= 1e-17) it = 99;  // one more evaluation at the converged node
+        }
+        x[i] = z;
+        x[n - 1 - i] = -z;
+        w[i] = w[n - 1 - i] = 2.0 / (omz2 * dp * dp);
+    }
+    if (n % 2) x[nh - 1] = 0.0;
+}
+
+// Orthonormal, no Condon-Shortley phase: P_lm, dP_lm/dtheta and m P_lm/sin(theta)
+// at the northern nodes, [lm][nhp] with zero padding.
+void legendre_tables(int T, int nh, int nhp, const std::vector<double> &ct,
+                     std::vector<double> &P, std::vector<double> &D, std::vector<double> &Q)
+{
+    const int nlm = (T + 1) * (T + 2) / 2;
+    P.assign((size_t)nlm * nhp, 0.0);
+    D.assign((size_t)nlm * nhp, 0.0);
+    Q.assign((size_t)nlm * nhp, 0.0);
+    for (int j = 0; j < nh; ++j) {
+        const double x = ct[j];
+        const double s = std::sqrt((1.0 - x) * (1.0 + x));
+        int lm = 0;
+        double pmm = 1.0 / std::sqrt(4.0 * kPi);
+        for (int m = 0; m <= T; ++m) {
+            if (m > 0) pmm *= std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * s;
+            double pm2 = 0.0, pm1 = 0.0;
+            for (int l = m; l <= T; ++l, ++lm) {
+                double p;
+                if (l == m) p = pmm;
+                else if (l == m + 1) p = std::sqrt(2.0 * m + 3.0) * x * pm1;
+                else {
+                    const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                    const double b = std::sqrt((((double)l - 1) * (l - 1) - (double)m * m) /
+                                               (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                    p = a * (x * pm1 - b * pm2);
+                }
+                const double c = (l > m) ? std::sqrt((2.0 * l + 1.0) / (2.0 * l - 1.0) *
+                                                     ((double)l * l - (double)m * m)) : 0.0;
+                P[(size_t)lm * nhp + j] = p;
+                D[(size_t)lm * nhp + j] = (l * x * p - c * pm1) / s;
+                Q[(size_t)lm * nhp + j] = m * p / s;
+                pm2 = pm1;
+                pm1 = p;
+            }
+        }
+    }
+}
+
+int upload(DevBuf &b, const void *src, size_t bytes)
+{
+    int rc = b.ensure(bytes);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return AER_OK;
+}
+
+template <class T> std::vector<T> convert(const std::vector<double> &v)
+{
+    return std::vector<T>(v.begin(), v.end());
+}
+
+int check_device()
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(AER_ENODEV, std::string("no HIP device: ") + hipGetErrorString(e));
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return fail(AER_ENODEV, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(AER_ENODEV, std::string("device is ") + prop.gcnArchName + ", the library is built for gfx950 only");
+    return AER_OK;
+}
+
+int ensure_tables32(aer_sht *h)
+{
+    if (h->tab32[0].p) return AER_OK;
+    std::vector<double> P, D, Q;
+    legendre_tables(h->lmax, h->nh, h->nhp, h->ct, P, D, Q);
+    const std::vector<double> *src[3] = {&P, &D, &Q};
+    for (int k = 0; k < 3; ++k) {
+        auto f = convert<float>(*src[k]);
+        int rc = upload(h->tab32[k], f.data(), f.size() * sizeof(float));
+        if (rc) return rc;
+    }
+    // twiddles
+    std::vector<double> tw64(h->tw64.bytes / sizeof(double));
+    HIP_TRY(hipMemcpy(tw64.data(), h->tw64.p, h->tw64.bytes, hipMemcpyDeviceToHost));
+    auto tw32 = convert<float>(tw64);
+    return upload(h->tw32, tw32.data(), tw32.size() * sizeof(float));
+}
+
+aer::CfgDev cfg_dev(const aer_sht *h, int prec)
+{
+    aer::CfgDev c{};
+    c.T = h->lmax;
+    c.nlat = h->nlat;
+    c.nlon = h->nlon;
+    c.nh = h->nh;
+    c.nhp = h->nhp;
+    c.npt = h->npt;
+    c.nlm = h->nlm;
+    c.pb = h->pb;
+    c.fb = h->fb;
+    const DevBuf *tab = (prec == AER_PREC_FP32) ? h->tab32 : h->tab64;
+    c.tabP = tab[0].p;
+    c.tabD = tab[1].p;
+    c.tabQ = tab[2].p;
+    c.lfac = (const double *)h->lfac.p;
+    c.wpair = (const double *)h->wpair.p;
+    c.tw = (prec == AER_PREC_FP32) ? h->tw32.p : h->tw64.p;
+    c.npass = (int)h->radix.size();
+    for (int p = 0; p < c.npass; ++p) {
+        c.radix[p] = h->radix[p];
+        c.twoff[p] = h->twoff[p];
+    }
+    return c;
+}
+
+// ------------------------------------------------------------------ job launch
+
+struct JobSpec {
+    int kind, nitem;
+    const double *sin0 = nullptr, *sin1 = nullptr;
+    double *sout0 = nullptr, *sout1 = nullptr;
+    double *gout0 = nullptr, *gout1 = nullptr;
+    const double *gin0 = nullptr, *gin1 = nullptr, *gin2 = nullptr;
+    long sstride = 0, gstride = 0;
+    int mode0 = aer::M_ONE, mode1 = aer::M_ONE, mode2 = aer::M_ZERO;
+    double gsc0 = 1.0, gsc1 = 1.0;
+    int accumulate = 0;
+};
+
+template <int PREC>
+int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
+{
+    using T = typename aer::Prec<PREC>::T;
+    aer::LaunchArgs L{};
+    L.njob = 0;
+    int ltask = 0, ftask = 0;
+    long four_elems = 0;
+    const aer::CfgDev C = cfg_dev(h, PREC);
+    const int ntpp = ((C.T + 2) / 2 + 15) / 16;
+    const int nsub = 8 / h->fb;
+    for (int k = 0; k < njob; ++k) {
+        const JobSpec &s = specs[k];
+        if (s.nitem <= 0) continue;
+        aer::Job &J = L.job[L.njob++];
+        J.c = C;
+        J.kind = s.kind;
+        J.nitem = s.nitem;
+        switch (s.kind) {
+            case aer::SYN_SCALAR: case aer::ANA_SCALAR: J.ngf = s.nitem; break;
+            case aer::SYN_VECTOR: case aer::SYN_GRAD: J.ngf = 2 * s.nitem; break;
+            default: J.ngf = (s.gin2 ? 3 : 2) * s.nitem; break;
+        }
+        J.ngroup = (J.ngf + 7) / 8;
+        if (synth) J.lgroups = J.ngroup;
+        else J.lgroups = (s.kind == aer::ANA_SCALAR) ? (s.nitem + 7) / 8 : (s.nitem + 3) / 4;
+        J.sstride = s.sstride;
+        J.gstride = s.gstride;
+        J.sin0 = s.sin0; J.sin1 = s.sin1;
+        J.sout0 = s.sout0; J.sout1 = s.sout1;
+        J.gout0 = s.gout0; J.gout1 = s.gout1;
+        J.gin0 = s.gin0; J.gin1 = s.gin1; J.gin2 = s.gin2;
+        J.mode0 = s.mode0; J.mode1 = s.mode1; J.mode2 = s.mode2;
+        J.gsc0 = s.gsc0; J.gsc1 = s.gsc1;
+        J.accumulate = s.accumulate;
+        J.four_off = four_elems;
+        four_elems += (long)J.ngroup * (C.T + 1) * C.nhp * 32;
+        J.ltask0 = ltask;
+        J.ftask0 = ftask;
+        if (synth) {
+            ltask += J.lgroups * (C.T + 1) * C.npt;
+            ftask += J.ngroup * nsub * ((C.nh + C.pb - 1) / C.pb);
+        } else {
+            ltask += J.lgroups * (C.T + 1) * 2 * ntpp;
+            ftask += J.ngroup * nsub * ((C.nhp + C.pb - 1) / C.pb);
+        }
+    }
+    if (L.njob == 0) return AER_OK;
+    int rc = h->four.ensure((size_t)four_elems * sizeof(T));
+    if (rc) return rc;
+    L.four = h->four.p;
+    const size_t lds = (size_t)2 * h->fb * h->pb * h->nlon * 2 * sizeof(T);
+    aer::LaunchArgs LL = L, LF = L;
+    LL.ntask = ltask;
+    LF.ntask = ftask;
+    if (synth) {
+        hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, h->stream, LL);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(aer::k_fft_synth<PREC>, dim3(ftask), dim3(256), lds, h->stream, LF);
+        HIP_TRY(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(aer::k_fft_anal<PREC>, dim3(ftask), dim3(256), lds, h->stream, LF);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, h->stream, LL);
+        HIP_TRY(hipGetLastError());
+    }
+    return AER_OK;
+}
+
+int run_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    if (njob > AER_MAXJOB) return fail(AER_EINVAL, "too many jobs in one launch");
+    switch (h->prec) {
+        case AER_PREC_FP64: return launch_jobs<0>(h, specs, njob, synth);
+        case AER_PREC_FP32: {
+            int rc = ensure_tables32(h);
+            if (rc) return rc;
+            return launch_jobs<1>(h, specs, njob, synth);
+        }
+        default: return fail(AER_EINVAL, "precision not available in this build");
+    }
+}
+
+long dense_s(const aer_sht *h) { return 2L * h->nlm; }
+long dense_g(const aer_sht *h) { return (long)h->nlon * h->nlat; }
+
+int check_batch(const aer_sht *h, int nfield, long qstride, long gstride)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    if (nfield < 0) return fail(AER_EINVAL, "nfield < 0");
+    if (qstride < dense_s(h)) return fail(AER_EINVAL, "spectral stride smaller than 2*nlm");
+    if (gstride < dense_g(h)) return fail(AER_EINVAL, "grid stride smaller than nlon*nlat");
+    return AER_OK;
+}
+
+int stage(aer_sht *h, int ns, int ng)
+{
+    int rc = h->stage_s.ensure((size_t)ns * dense_s(h) * sizeof(double));
+    if (rc) return rc;
+    return h->stage_g.ensure((size_t)ng * dense_g(h) * sizeof(double));
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char *aer_strerror(int status)
+{
+    switch (status) {
+        case AER_OK: return "success";
+        case AER_EINVAL: return "invalid argument";
+        case AER_ENOMEM: return "out of memory";
+        case AER_EHIP: return "HIP runtime error";
+        case AER_ENODEV: return "no gfx950 device";
+        case AER_ESTATE: return "call out of order";
+        default: return "unknown status";
+    }
+}
+
+const char *aer_last_error(void) { return g_last_error.c_str(); }
+
+int aer_sht_grid_size(int trunc, int *nlon, int *nlat)
+{
+    if (trunc < 0 || !nlon || !nlat) return fail(AER_EINVAL, "aer_sht_grid_size: bad argument");
+    int n = 3 * trunc + 1;
+    while (!smooth_even(n)) ++n;
+    *nlon = n;
+    *nlat = n / 2;
+    return AER_OK;
+}
+
+int aer_sht_create(int lmax, int mmax, int mres, int norm, aer_sht_t *cfg)
+{
+    if (!cfg) return fail(AER_EINVAL, "aer_sht_create: null output");
+    *cfg = nullptr;
+    if (lmax < 1 || mmax != lmax || mres != 1)
+        return fail(AER_EINVAL, "aer_sht_create: only triangular lmax = mmax >= 1, mres = 1 is supported");
+    if ((norm & (AER_SHT_ORTHONORMAL | AER_SHT_NO_CS_PHASE)) != (AER_SHT_ORTHONORMAL | AER_SHT_NO_CS_PHASE))
+        return fail(AER_EINVAL, "aer_sht_create: norm must be AER_SHT_ORTHONORMAL | AER_SHT_NO_CS_PHASE");
+    aer_sht *h = new aer_sht();
+    h->lmax = lmax;
+    h->mmax = mmax;
+    h->mres = mres;
+    h->norm = norm;
+    h->nlm = (lmax + 1) * (lmax + 2) / 2;
+    for (int m = 0; m <= mmax; ++m)
+        for (int l = m; l <= lmax; ++l) {
+            h->li.push_back(l);
+            h->mi.push_back(m);
+        }
+    *cfg = h;
+    return AER_OK;
+}
+
+int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
+{
+    if (!h) return fail(AER_EINVAL, "aer_sht_set_grid: null handle");
+    if (!(layout & AER_SHT_PHI_CONTIGUOUS)) return fail(AER_EINVAL, "aer_sht_set_grid: only AER_SHT_PHI_CONTIGUOUS is supported");
+    if (nlat < h->lmax + 1) return fail(AER_EINVAL, "aer_sht_set_grid: nlat < lmax+1 is not exact");
+    if (nphi <= 2 * h->mmax) return fail(AER_EINVAL, "aer_sht_set_grid: nphi <= 2*mmax");
+    {
+        int n = nphi;
+        while (n % 2 == 0) n /= 2;
+        while (n % 3 == 0) n /= 3;
+        while (n % 5 == 0) n /= 5;
+        if (n != 1) return fail(AER_EINVAL, "aer_sht_set_grid: nphi must factor into 2, 3 and 5");
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    h->layout = layout;
+    h->eps = eps;
+    h->nlat = nlat;
+    h->nlon = nphi;
+    h->nh = (nlat + 1) / 2;
+    h->nhp = (h->nh + 15) / 16 * 16;
+    h->npt = h->nhp / 16;
+    // FFT block: fb grid fields x pb pairs, at most 2048 complex per LDS buffer
+    h->fb = 8;
+    h->pb = std::max(1, 256 / nphi);
+    while (h->fb * h->pb * nphi > 2048 && h->fb > 1) h->fb /= 2;
+    gauss_legendre(nlat, h->ct, h->w);
+    std::vector<double> P, D, Q;
+    legendre_tables(h->lmax, h->nh, h->nhp, h->ct, P, D, Q);
+    const std::vector<double> *tabs[3] = {&P, &D, &Q};
+    for (int k = 0; k < 3; ++k)
+        if ((rc = upload(h->tab64[k], tabs[k]->data(), tabs[k]->size() * sizeof(double)))) return rc;
+    for (auto &b : h->tab32) b.release();
+    // per-degree factors
+    const int T = h->lmax;
+    std::vector<double> lf((size_t)AER_NMODE * (T + 1), 0.0);
+    for (int l = 0; l <= T; ++l) {
+        const double ll1 = (double)(l * (l + 1));
+        lf[aer::M_ONE * (T + 1) + l] = 1.0;
+        lf[aer::M_S_FROM_DIV * (T + 1) + l] = l ? -(kREarth / ll1) : 0.0;
+        lf[aer::M_T_FROM_VOR * (T + 1) + l] = l ? (kREarth / ll1) : 0.0;
+        lf[aer::M_INV_LL1 * (T + 1) + l] = l ? 1.0 / ll1 : 0.0;
+        lf[aer::M_DIV_FROM_S * (T + 1) + l] = l ? -1.0 / kREarth : 0.0;
+        lf[aer::M_VOR_FROM_T * (T + 1) + l] = l ? 1.0 / kREarth : 0.0;
+        lf[aer::M_NEG_LAP * (T + 1) + l] = ll1 / (kREarth * kREarth);
+        lf[aer::M_ZERO * (T + 1) + l] = 0.0;
+    }
+    if ((rc = upload(h->lfac, lf.data(), lf.size() * sizeof(double)))) return rc;
+    std::vector<double> wp(h->nhp, 0.0);
+    for (int j = 0; j < h->nh; ++j) {
+        wp[j] = h->w[j] * 2.0 * kPi / nphi;
+        if (nlat - 1 - j == j) wp[j] *= 0.5;
+    }
+    if ((rc = upload(h->wpair, wp.data(), wp.size() * sizeof(double)))) return rc;
+    // FFT plan: radices 4, 2, 3, 5 and the forward twiddles of each pass
+    h->radix.clear();
+    h->twoff.clear();
+    {
+        int n = nphi;
+        while (n % 4 == 0) { h->radix.push_back(4); n /= 4; }
+        while (n % 2 == 0) { h->radix.push_back(2); n /= 2; }
+        while (n % 3 == 0) { h->radix.push_back(3); n /= 3; }
+        while (n % 5 == 0) { h->radix.push_back(5); n /= 5; }
+    }
+    if ((int)h->radix.size() > AER_MAXPASS) return fail(AER_EINVAL, "aer_sht_set_grid: nphi needs too many FFT passes");
+    std::vector<double> tw;
+    int Ns = 1;
+    for (int R : h->radix) {
+        h->twoff.push_back((int)(tw.size() / 2));
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                const double a = -2.0 * kPi * r * k / ((double)Ns * R);
+                tw.push_back(std::cos(a));
+                tw.push_back(std::sin(a));
+            }
+        Ns *= R;
+    }
+    if (tw.empty()) tw.assign(2, 0.0);
+    if ((rc = upload(h->tw64, tw.data(), tw.size() * sizeof(double)))) return rc;
+    h->tw32.release();
+    h->grid_set = true;
+    return AER_OK;
+}
+
+int aer_sht_get_info(aer_sht_t h, aer_sht_info *info)
+{
+    if (!h || !info) return fail(AER_EINVAL, "aer_sht_get_info: null argument");
+    info->lmax = h->lmax;
+    info->mmax = h->mmax;
+    info->mres = h->mres;
+    info->nlm = h->nlm;
+    info->nlat = h->nlat;
+    info->nphi = h->nlon;
+    info->nspat = h->nlat * h->nlon;
+    info->nlat_2 = (h->nlat + 1) / 2;
+    return AER_OK;
+}
+
+int aer_sht_get_lm(aer_sht_t h, int *li, int *mi)
+{
+    if (!h || !li || !mi) return fail(AER_EINVAL, "aer_sht_get_lm: null argument");
+    std::memcpy(li, h->li.data(), sizeof(int) * h->nlm);
+    std::memcpy(mi, h->mi.data(), sizeof(int) * h->nlm);
+    return AER_OK;
+}
+
+int aer_sht_get_cos_theta(aer_sht_t h, double *ct)
+{
+    if (!h || !ct) return fail(AER_EINVAL, "aer_sht_get_cos_theta: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "aer_sht_get_cos_theta before set_grid");
+    std::memcpy(ct, h->ct.data(), sizeof(double) * h->nlat);
+    return AER_OK;
+}
+
+int aer_sht_gauss_wts(aer_sht_t h, double *wts_half)
+{
+    if (!h || !wts_half) return fail(AER_EINVAL, "aer_sht_gauss_wts: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "aer_sht_gauss_wts before set_grid");
+    std::memcpy(wts_half, h->w.data(), sizeof(double) * ((h->nlat + 1) / 2));
+    return AER_OK;
+}
+
+int aer_sht_destroy(aer_sht_t h)
+{
+    if (!h) return AER_OK;
+    for (auto &b : h->tab64) b.release();
+    for (auto &b : h->tab32) b.release();
+    h->lfac.release();
+    h->wpair.release();
+    h->tw64.release();
+    h->tw32.release();
+    h->four.release();
+    h->stage_s.release();
+    h->stage_g.release();
+    delete h;
+    return AER_OK;
+}
+
+int aer_sht_set_precision(aer_sht_t h, int prec)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (prec != AER_PREC_FP64 && prec != AER_PREC_FP32)
+        return fail(AER_EINVAL, "aer_sht_set_precision: unsupported precision");
+    h->prec = prec;
+    return AER_OK;
+}
+
+int aer_sht_set_stream(aer_sht_t h, void *stream)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    h->stream = (hipStream_t)stream;
+    return AER_OK;
+}
+
+int aer_sht_sync(aer_sht_t h)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+// ---- batched device transforms
+
+int aer_synth_d(aer_sht_t h, int nfield, const double *qlm, long qstride, double *vr, long gstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_SCALAR, nfield};
+    s.sin0 = qlm; s.sstride = qstride; s.gout0 = vr; s.gstride = gstride;
+    return run_jobs(h, &s, 1, true);
+}
+
+int aer_anal_d(aer_sht_t h, int nfield, const double *vr, long gstride, double *qlm, long qstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::ANA_SCALAR, nfield};
+    s.gin0 = vr; s.gstride = gstride; s.sout0 = qlm; s.sstride = qstride;
+    return run_jobs(h, &s, 1, false);
+}
+
+int aer_synth_vec_d(aer_sht_t h, int nfield, const double *slm, const double *tlm, long qstride,
+                    double *vt, double *vp, long gstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_VECTOR, nfield};
+    s.sin0 = slm; s.sin1 = tlm; s.sstride = qstride;
+    s.gout0 = vt; s.gout1 = vp; s.gstride = gstride;
+    return run_jobs(h, &s, 1, true);
+}
+
+int aer_anal_vec_d(aer_sht_t h, int nfield, const double *vt, const double *vp, long gstride,
+                   double *slm, double *tlm, long qstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::ANA_VECTOR, nfield};
+    s.gin0 = vt; s.gin1 = vp; s.gstride = gstride;
+    s.sout0 = slm; s.sout1 = tlm; s.sstride = qstride;
+    s.mode0 = s.mode1 = aer::M_INV_LL1;
+    return run_jobs(h, &s, 1, false);
+}
+
+int aer_synth_grad_d(aer_sht_t h, int nfield, const double *slm, long qstride,
+                     double *gt, double *gp, long gstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_GRAD, nfield};
+    s.sin0 = slm; s.sstride = qstride; s.gout0 = gt; s.gout1 = gp; s.gstride = gstride;
+    return run_jobs(h, &s, 1, true);
+}
+
+// ---- aeros_vordiv
+
+int aer_uv_from_vordiv_d(aer_sht_t h, int nlev, const double *vor, const double *div, double *u, double *v)
+{
+    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_VECTOR, nlev};
+    s.sin0 = div; s.mode0 = aer::M_S_FROM_DIV;
+    s.sin1 = vor; s.mode1 = aer::M_T_FROM_VOR;
+    s.sstride = dense_s(h);
+    s.gout0 = v; s.gsc0 = -1.0;  // v = -V_theta
+    s.gout1 = u; s.gsc1 = 1.0;   // u = V_phi
+    s.gstride = dense_g(h);
+    return run_jobs(h, &s, 1, true);
+}
+
+int aer_vordiv_from_uv_d(aer_sht_t h, int nlev, const double *u, const double *v, double *vor, double *div)
+{
+    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    if (rc) return rc;
+    JobSpec s{aer::ANA_VECTOR, nlev};
+    s.gin0 = v; s.gsc0 = -1.0;   // V_theta = -v
+    s.gin1 = u; s.gsc1 = 1.0;    // V_phi = u
+    s.gstride = dense_g(h);
+    s.sout0 = div; s.mode0 = aer::M_DIV_FROM_S;
+    s.sout1 = vor; s.mode1 = aer::M_VOR_FROM_T;
+    s.sstride = dense_s(h);
+    return run_jobs(h, &s, 1, false);
+}
+
+int aer_gradient_d(aer_sht_t h, int nlev, const double *f, double *dfdx, double *dfdy)
+{
+    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_GRAD, nlev};
+    s.sin0 = f; s.sstride = dense_s(h);
+    s.gout0 = dfdy; s.gsc0 = -1.0 / kREarth;  // north = -g_theta/a
+    s.gout1 = dfdx; s.gsc1 = 1.0 / kREarth;   // east  =  g_phi/a
+    s.gstride = dense_g(h);
+    return run_jobs(h, &s, 1, true);
+}
+
+// ---- tendency transform halves
+
+int aer_tendency_grid_transforms_d(aer_sht_t h, int nlev,
+        const double *vor, const double *div, const double *temp, const double *lnps,
+        double *u, double *v, double *vor_g, double *div_g, double *t_g,
+        double *dtdx, double *dtdy, double *lnps_g, double *dlnpsdx, double *dlnpsdy)
+{
+    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    if (rc) return rc;
+    const long ss = dense_s(h), gs = dense_g(h);
+    JobSpec j[7] = {{aer::SYN_SCALAR, 1}, {aer::SYN_GRAD, 1}, {aer::SYN_VECTOR, nlev},
+                    {aer::SYN_SCALAR, nlev}, {aer::SYN_SCALAR, nlev}, {aer::SYN_SCALAR, nlev},
+                    {aer::SYN_GRAD, nlev}};
+    for (auto &s : j) { s.sstride = ss; s.gstride = gs; }
+    j[0].sin0 = lnps; j[0].gout0 = lnps_g;
+    j[1].sin0 = lnps; j[1].gout0 = dlnpsdy; j[1].gsc0 = -1.0 / kREarth; j[1].gout1 = dlnpsdx; j[1].gsc1 = 1.0 / kREarth;
+    j[2].sin0 = div; j[2].mode0 = aer::M_S_FROM_DIV; j[2].sin1 = vor; j[2].mode1 = aer::M_T_FROM_VOR;
+    j[2].gout0 = v; j[2].gsc0 = -1.0; j[2].gout1 = u;
+    j[3].sin0 = vor; j[3].gout0 = vor_g;
+    j[4].sin0 = div; j[4].gout0 = div_g;
+    j[5].sin0 = temp; j[5].gout0 = t_g;
+    j[6].sin0 = temp; j[6].gout0 = dtdy; j[6].gsc0 = -1.0 / kREarth; j[6].gout1 = dtdx; j[6].gsc1 = 1.0 / kREarth;
+    return run_jobs(h, j, 7, true);
+}
+
+int aer_tendency_spectral_d(aer_sht_t h, int nlev,
+        const double *ae, const double *an, const double *ek, const double *dtdt,
+        const double *dlnpsdt, double *tvor, double *tdiv, double *ttemp, double *tlnps)
+{
+    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    if (rc) return rc;
+    const long ss = dense_s(h), gs = dense_g(h);
+    JobSpec j[3] = {{aer::ANA_VECTOR, nlev}, {aer::ANA_SCALAR, nlev}, {aer::ANA_SCALAR, 1}};
+    for (auto &s : j) { s.sstride = ss; s.gstride = gs; }
+    j[0].gin0 = an; j[0].gsc0 = -1.0; j[0].gin1 = ae; j[0].gin2 = ek;
+    j[0].sout0 = tdiv; j[0].mode0 = aer::M_DIV_FROM_S;
+    j[0].sout1 = tvor; j[0].mode1 = aer::M_VOR_FROM_T;
+    j[0].mode2 = aer::M_NEG_LAP;
+    j[1].gin0 = dtdt; j[1].sout0 = ttemp;
+    j[2].gin0 = dlnpsdt; j[2].sout0 = tlnps;
+    return run_jobs(h, j, 3, false);
+}
+
+// ---- per-call host API (one field, synchronous)
+
+int aer_SH_to_spat(aer_sht_t h, const double *qlm, double *vr)
+{
+    if (!h || !qlm || !vr) return fail(AER_EINVAL, "aer_SH_to_spat: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    int rc = stage(h, 1, 1);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h->stage_s.p, qlm, dense_s(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = aer_synth_d(h, 1, (const double *)h->stage_s.p, dense_s(h), (double *)h->stage_g.p, dense_g(h)))) return rc;
+    HIP_TRY(hipMemcpyAsync(vr, h->stage_g.p, dense_g(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+int aer_spat_to_SH(aer_sht_t h, double *vr, double *qlm)
+{
+    if (!h || !qlm || !vr) return fail(AER_EINVAL, "aer_spat_to_SH: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    int rc = stage(h, 1, 1);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h->stage_g.p, vr, dense_g(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = aer_anal_d(h, 1, (const double *)h->stage_g.p, dense_g(h), (double *)h->stage_s.p, dense_s(h)))) return rc;
+    HIP_TRY(hipMemcpyAsync(qlm, h->stage_s.p, dense_s(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+int aer_SHsphtor_to_spat(aer_sht_t h, const double *slm, const double *tlm, double *vt, double *vp)
+{
+    if (!h || !slm || !tlm || !vt || !vp) return fail(AER_EINVAL, "aer_SHsphtor_to_spat: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    int rc = stage(h, 2, 2);
+    if (rc) return rc;
+    double *ds = (double *)h->stage_s.p, *dg = (double *)h->stage_g.p;
+    const long ss = dense_s(h), gs = dense_g(h);
+    HIP_TRY(hipMemcpyAsync(ds, slm, ss * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(ds + ss, tlm, ss * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = aer_synth_vec_d(h, 1, ds, ds + ss, ss, dg, dg + gs, gs))) return rc;
+    HIP_TRY(hipMemcpyAsync(vt, dg, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(vp, dg + gs, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+int aer_spat_to_SHsphtor(aer_sht_t h, double *vt, double *vp, double *slm, double *tlm)
+{
+    if (!h || !slm || !tlm || !vt || !vp) return fail(AER_EINVAL, "aer_spat_to_SHsphtor: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    int rc = stage(h, 2, 2);
+    if (rc) return rc;
+    double *ds = (double *)h->stage_s.p, *dg = (double *)h->stage_g.p;
+    const long ss = dense_s(h), gs = dense_g(h);
+    HIP_TRY(hipMemcpyAsync(dg, vt, gs * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dg + gs, vp, gs * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = aer_anal_vec_d(h, 1, dg, dg + gs, gs, ds, ds + ss, ss))) return rc;
+    HIP_TRY(hipMemcpyAsync(slm, ds, ss * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(tlm, ds + ss, ss * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+int aer_SHsph_to_spat(aer_sht_t h, const double *slm, double *vt, double *vp)
+{
+    if (!h || !slm || !vt || !vp) return fail(AER_EINVAL, "aer_SHsph_to_spat: null argument");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    int rc = stage(h, 1, 2);
+    if (rc) return rc;
+    double *ds = (double *)h->stage_s.p, *dg = (double *)h->stage_g.p;
+    const long ss = dense_s(h), gs = dense_g(h);
+    HIP_TRY(hipMemcpyAsync(ds, slm, ss * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = aer_synth_grad_d(h, 1, ds, ss, dg, dg + gs, gs))) return rc;
+    HIP_TRY(hipMemcpyAsync(vt, dg, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(vp, dg + gs, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+/*
+ * aer_sht.h -- C ABI of the MI355X (gfx950) spherical-harmonic transform
+ * library that replaces SHTns on the aeros transform path.
+ *
+ * The reference reaches SHTns from Fortran through SHTns' own iso_c_binding
+ * header (`include 'shtns.f03'`, reference src/aeros_spectral.f90:41-45). Each
+ * entry point below names the SHTns call it replaces and the reference line
+ * that makes that call. The Fortran module aeros_amd/fortran/aeros_spectral.f90
+ * binds these with iso_c_binding and keeps the reference's module API.
+ *
+ * Conventions (reference src/aeros_spectral.f90:17-26): orthonormal harmonics,
+ * no Condon-Shortley phase; grids are (nlon, nlat) Fortran order, longitude
+ * contiguous, latitude north -> south, no padding; spectral arrays are nlm
+ * complex coefficients, m-major with l fastest, stored as interleaved
+ * (re, im) doubles (Fortran complex(c_double_complex)); triangular
+ * truncation lmax = mmax, mres = 1.
+ *
+ * Every function returns an int status: 0 on success, an AER_E* code
+ * otherwise. aer_strerror() gives a message; aer_last_error() gives the detail
+ * of the most recent failure on the calling thread. The Fortran shim turns a
+ * non-zero status into a message on error_unit plus `error stop`, as the
+ * reference does for its own fatal conditions (src/aeros_spectral.f90:236-244).
+ *
+ * Threading: a handle is not safe for concurrent calls (the reference's SHTns
+ * configs are not either, src/aeros_spectral.f90:50-54); create one handle
+ * per host thread. Results are deterministic: no atomics, fixed reduction
+ * order, so repeated calls are bit-identical.
+ */
+#ifndef AER_SHT_H
+#define AER_SHT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+#define AER_OK      0
+#define AER_EINVAL  1  /* invalid argument, truncation, grid or flag combination */
+#define AER_ENOMEM  2  /* device or host allocation failed */
+#define AER_EHIP    3  /* HIP runtime error (detail in aer_last_error) */
+#define AER_ENODEV  4  /* no gfx950 device visible: there is no CPU fallback */
+#define AER_ESTATE  5  /* call out of order, e.g. a transform before set_grid */
+
+/* ---- normalization / layout flags: the subset aeros requests
+ * (SHT_ORTHONORMAL + SHT_NO_CS_PHASE, SHT_PHI_CONTIGUOUS, SHT_GAUSS or
+ * SHT_QUICK_INIT, optional SHT_LOAD_SAVE_CFG; src/aeros_spectral.f90:200-221). */
+#define AER_SHT_ORTHONORMAL    0x1
+#define AER_SHT_NO_CS_PHASE    0x2
+#define AER_SHT_PHI_CONTIGUOUS 0x10
+#define AER_SHT_GAUSS          0x20
+#define AER_SHT_QUICK_INIT     0x40
+#define AER_SHT_LOAD_SAVE_CFG  0x80
+
+/* ---- arithmetic of the Legendre contraction and the longitude FFT */
+#define AER_PREC_FP64  0  /* FP64 MFMA Legendre GEMM, FP64 FFT (the plain form) */
+#define AER_PREC_FP32  1  /* FP32 MFMA Legendre GEMM, FP32 FFT */
+#define AER_PREC_SPLIT 2  /* split-FP32 MFMA Legendre GEMM (near-FP64), FP64 FFT */
+
+typedef struct aer_sht *aer_sht_t;
+
+typedef struct aer_sht_info {
+    int lmax, mmax, mres;
+    int nlm;      /* (lmax+1)(lmax+2)/2 */
+    int nlat, nphi, nspat, nlat_2;  /* nspat = nphi*nlat (never padded); nlat_2 = (nlat+1)/2 */
+} aer_sht_info;
+
+const char *aer_strerror(int status);
+const char *aer_last_error(void);
+
+/* Smallest quadratically unaliased Gaussian grid for truncation T: nlon >= 3T+1,
+ * 2/3/5-smooth and even, nlat = nlon/2. Replaces aeros_sht_grid_size
+ * (src/aeros_spectral.f90:399-438). */
+int aer_sht_grid_size(int trunc, int *nlon, int *nlat);
+
+/* Replaces shtns_create (src/aeros_spectral.f90:225). Host-only; norm must be
+ * AER_SHT_ORTHONORMAL | AER_SHT_NO_CS_PHASE, mres must be 1, mmax == lmax. */
+int aer_sht_create(int lmax, int mmax, int mres, int norm, aer_sht_t *cfg);
+
+/* Replaces shtns_set_grid (src/aeros_spectral.f90:227; NB SHTns' argument order
+ * nlat, nphi). Builds the Gauss grid and the FP64 Legendre tables and uploads
+ * them to the current HIP device. layout must contain AER_SHT_PHI_CONTIGUOUS;
+ * SHT_GAUSS / QUICK_INIT / LOAD_SAVE_CFG are accepted and change nothing (there
+ * is no auto-tuning step). eps is recorded only: the contraction is always the
+ * full dense sum (SHTns' polar skip below eps is not applied). */
+int aer_sht_set_grid(aer_sht_t cfg, int layout, double eps, int nlat, int nphi);
+
+/* Read-back of the realized configuration (the shtns_info fields read at
+ * src/aeros_spectral.f90:230-260). */
+int aer_sht_get_info(aer_sht_t cfg, aer_sht_info *info);
+/* li[lm], mi[lm] for lm = 0..nlm-1 (SHTns info%li / info%mi, src/aeros_spectral.f90:246-252). */
+int aer_sht_get_lm(aer_sht_t cfg, int *li, int *mi);
+/* cos(colatitude) of the Gauss nodes, north -> south (info%ct, src/aeros_spectral.f90:255-258). */
+int aer_sht_get_cos_theta(aer_sht_t cfg, double *ct);
+/* Gauss weights of the northern half, nlat_2 values (shtns_gauss_wts, src/aeros_spectral.f90:271). */
+int aer_sht_gauss_wts(aer_sht_t cfg, double *wts_half);
+/* Replaces shtns_destroy (src/aeros_spectral.f90:288). Accepts NULL. */
+int aer_sht_destroy(aer_sht_t cfg);
+
+/* Precision of subsequent transforms on this handle (default AER_PREC_FP64). */
+int aer_sht_set_precision(aer_sht_t cfg, int prec);
+/* HIP stream (hipStream_t) for subsequent device-pointer calls; NULL = default stream. */
+int aer_sht_set_stream(aer_sht_t cfg, void *stream);
+
+/* ---- per-call transforms on HOST arrays, one field; synchronous.
+ * Drop-in replacements for the five SHTns calls aeros makes. Analyses leave
+ * their grid inputs unchanged (the reference allows them to be destroyed,
+ * src/aeros_spectral.f90:125-126). */
+int aer_spat_to_SH(aer_sht_t cfg, double *vr, double *qlm);                    /* spat_to_SH,       src/aeros_spectral.f90:496 */
+int aer_SH_to_spat(aer_sht_t cfg, const double *qlm, double *vr);              /* SH_to_spat,       src/aeros_spectral.f90:511 */
+int aer_spat_to_SHsphtor(aer_sht_t cfg, double *vt, double *vp,
+                         double *slm, double *tlm);                             /* spat_to_SHsphtor, src/aeros_spectral.f90:539 */
+int aer_SHsphtor_to_spat(aer_sht_t cfg, const double *slm, const double *tlm,
+                         double *vt, double *vp);                               /* SHsphtor_to_spat, src/aeros_spectral.f90:554 */
+int aer_SHsph_to_spat(aer_sht_t cfg, const double *slm, double *vt, double *vp);/* SHsph_to_spat,    src/aeros_spectral.f90:580 */
+
+/* ---- batched transforms on DEVICE arrays (asynchronous on the handle's stream).
+ * nfield fields; field k of a spectral array starts at q + k*qstride doubles
+ * (qstride >= 2*nlm), of a grid array at g + k*gstride doubles
+ * (gstride >= nlon*nlat). These are the level loops of the reference
+ * (src/dynamics/aeros_tendency.f90:409-419, 449-465) as one launch. */
+int aer_synth_d(aer_sht_t cfg, int nfield, const double *qlm, long qstride, double *vr, long gstride);
+int aer_anal_d(aer_sht_t cfg, int nfield, const double *vr, long gstride, double *qlm, long qstride);
+int aer_synth_vec_d(aer_sht_t cfg, int nfield, const double *slm, const double *tlm, long qstride,
+                    double *vt, double *vp, long gstride);
+int aer_anal_vec_d(aer_sht_t cfg, int nfield, const double *vt, const double *vp, long gstride,
+                   double *slm, double *tlm, long qstride);
+int aer_synth_grad_d(aer_sht_t cfg, int nfield, const double *slm, long qstride,
+                     double *gt, double *gp, long gstride);
+
+/* ---- aeros_vordiv on DEVICE arrays, nlev levels, dense (nlm | nlon*nlat per level).
+ * uv_from_vordiv:  src/dynamics/aeros_vordiv.f90:82-122
+ * vordiv_from_uv:  src/dynamics/aeros_vordiv.f90:124-168 (inputs preserved)
+ * gradient:        src/dynamics/aeros_vordiv.f90:170-205 (east, north components) */
+int aer_uv_from_vordiv_d(aer_sht_t cfg, int nlev, const double *vor, const double *div,
+                         double *u, double *v);
+int aer_vordiv_from_uv_d(aer_sht_t cfg, int nlev, const double *u, const double *v,
+                         double *vor, double *div);
+int aer_gradient_d(aer_sht_t cfg, int nlev, const double *f, double *dfdx, double *dfdy);
+
+/* ---- the transform halves of the tendency, DEVICE arrays, all levels fused.
+ * 3-D spectral arrays are (nlm, nlev), 3-D grid arrays (nlon, nlat, nlev), Fortran order.
+ * grid_transforms: the spectral->grid half of aeros_tendency_grid
+ *                  (src/dynamics/aeros_tendency.f90:403-419)
+ * spectral:        aeros_tendency_spectral, the spectral tendency assembly
+ *                  (src/dynamics/aeros_tendency.f90:428-473): (zeta_t, D_t) from the
+ *                  vector analysis of (ae, an), D_t -= laplacian(analysis(ek)),
+ *                  T_t = analysis(dtdt), lnps_t = analysis(dlnpsdt). Inputs preserved. */
+int aer_tendency_grid_transforms_d(aer_sht_t cfg, int nlev,
+        const double *vor, const double *div, const double *temp, const double *lnps,
+        double *u, double *v, double *vor_g, double *div_g, double *t_g,
+        double *dtdx, double *dtdy, double *lnps_g, double *dlnpsdx, double *dlnpsdy);
+int aer_tendency_spectral_d(aer_sht_t cfg, int nlev,
+        const double *ae, const double *an, const double *ek, const double *dtdt,
+        const double *dlnpsdt,
+        double *tvor, double *tdiv, double *ttemp, double *tlnps);
+
+/* Block until all work queued on the handle's stream has finished. */
+int aer_sht_sync(aer_sht_t cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,28 @@
+"""Build recipe for the FP64 CPU oracle (test infrastructure only).
+
+Compiles oracle/sht_oracle.c with gcc and OpenMP into oracle/liborc.so. No
+reference build exists for this path: the reference's transform arithmetic is
+the third-party SHTns C library, which the reference does not vendor, so there
+is no oracle/_ref target.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+
+def build(verbose: bool = False) -> str:
+    src = os.path.join(HERE, "sht_oracle.c")
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c11",
+           "-Wall", "-Wno-unused-function", src, "-o", LIB, "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,246 @@
+"""numpy front end of the FP64 CPU oracle (oracle/sht_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py. The product path (aeros_amd) never imports it.
+
+Arrays: spectral fields are complex128 of shape (nlm,), m-major, l fastest;
+grid fields are float64 of shape (nlat, nlon), i.e. the reference's Fortran
+(nlon, nlat) column-major layout seen from C order (longitude contiguous,
+latitude north -> south).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("trunc", ctypes.c_int), ("lmax", ctypes.c_int), ("mmax", ctypes.c_int),
+                ("nlon", ctypes.c_int), ("nlat", ctypes.c_int), ("nlm", ctypes.c_int),
+                ("mu", _dp), ("w", _dp), ("sint", _dp), ("P", _dp), ("D", _dp),
+                ("l_of_lm", _ip), ("m_of_lm", _ip), ("fft_direct", ctypes.c_int),
+                ("tw_fwd", _dp), ("tw_inv", _dp)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            from oracle import build_oracle
+            build_oracle.build()
+        L = ctypes.CDLL(_LIBPATH)
+        cp = ctypes.POINTER(_Cfg)
+        L.orc_create.restype = cp
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_destroy.argtypes = [cp]
+        L.orc_set_fft_direct.argtypes = [cp, ctypes.c_int]
+        L.orc_grid_size.argtypes = [ctypes.c_int, _ip, _ip]
+        L.orc_lm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_gauss.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_rfft_forward.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_rfft_inverse.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        for name, n in [("orc_synth", 2), ("orc_anal", 2), ("orc_synth_grad", 3),
+                        ("orc_synth_vec", 4), ("orc_anal_vec", 4),
+                        ("orc_uv_from_vordiv", 4), ("orc_vordiv_from_uv", 4),
+                        ("orc_gradient", 3)]:
+            getattr(L, name).argtypes = [cp] + [_dp] * n
+        L.orc_laplacian.argtypes = [cp, _dp, ctypes.c_int]
+        L.orc_surface_integral.argtypes = [cp, _dp]
+        L.orc_surface_integral.restype = ctypes.c_double
+        L.orc_tendency_grid_transforms.argtypes = [cp, ctypes.c_int] + [_dp] * 14
+        L.orc_tendency_spectral.argtypes = [cp, ctypes.c_int] + [_dp] * 9
+        L.orc_bench_set.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_bench_set.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def grid_size(trunc):
+    a, b = ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_grid_size(trunc, ctypes.byref(a), ctypes.byref(b))
+    if rc:
+        raise ValueError("invalid truncation")
+    return a.value, b.value
+
+
+def lm_index(lmax, l, m):
+    """1-based (l, m) index, -1 outside the truncation."""
+    return lib().orc_lm(lmax, l, m)
+
+
+def gauss(n):
+    mu = np.empty(n)
+    w = np.empty(n)
+    lib().orc_gauss(n, _p(mu), _p(w))
+    return mu, w
+
+
+def legendre(lmax, mu):
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    nlm = (lmax + 1) * (lmax + 2) // 2
+    P = np.empty((nlm, mu.size))
+    D = np.empty((nlm, mu.size))
+    lib().orc_legendre(lmax, mu.size, _p(mu), _p(P), _p(D))
+    return P, D
+
+
+def rfft_forward(x, mmax):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    X = np.empty(mmax + 1, dtype=np.complex128)
+    lib().orc_rfft_forward(x.size, mmax, _p(x), X.ctypes.data_as(_dp))
+    return X
+
+
+def rfft_inverse(X, n):
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    x = np.empty(n)
+    lib().orc_rfft_inverse(n, X.size - 1, X.ctypes.data_as(_dp), _p(x))
+    return x
+
+
+class Oracle:
+    """One truncation and grid, with the reference's per-call transform API."""
+
+    def __init__(self, trunc, nlon=0, nlat=0, fft_direct=False):
+        self._c = lib().orc_create(trunc, nlon, nlat)
+        if not self._c:
+            raise ValueError("invalid truncation")
+        c = self._c.contents
+        self.trunc, self.lmax, self.mmax = c.trunc, c.lmax, c.mmax
+        self.nlon, self.nlat, self.nlm = c.nlon, c.nlat, c.nlm
+        self.mu = np.ctypeslib.as_array(c.mu, (self.nlat,)).copy()
+        self.w = np.ctypeslib.as_array(c.w, (self.nlat,)).copy()
+        self.P = np.ctypeslib.as_array(c.P, (self.nlm, self.nlat)).copy()
+        self.D = np.ctypeslib.as_array(c.D, (self.nlm, self.nlat)).copy()
+        self.l_of_lm = np.ctypeslib.as_array(c.l_of_lm, (self.nlm,)).copy()
+        self.m_of_lm = np.ctypeslib.as_array(c.m_of_lm, (self.nlm,)).copy()
+        if fft_direct:
+            lib().orc_set_fft_direct(self._c, 1)
+
+    def __del__(self):
+        if getattr(self, "_c", None) and _lib is not None:
+            _lib.orc_destroy(self._c)
+            self._c = None
+
+    # --- helpers
+    def _spec(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == (self.nlm,), a.shape
+        return a
+
+    def _grid(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.nlat, self.nlon), a.shape
+        return a
+
+    def _g(self):
+        return np.empty((self.nlat, self.nlon))
+
+    def _s(self):
+        return np.empty(self.nlm, dtype=np.complex128)
+
+    # --- transforms
+    def synth(self, q):
+        q = self._spec(q)
+        g = self._g()
+        lib().orc_synth(self._c, q.ctypes.data_as(_dp), _p(g))
+        return g
+
+    def anal(self, g):
+        g = self._grid(g)
+        q = self._s()
+        lib().orc_anal(self._c, _p(g), q.ctypes.data_as(_dp))
+        return q
+
+    def synth_vec(self, s, t):
+        s, t = self._spec(s), self._spec(t)
+        vt, vp = self._g(), self._g()
+        lib().orc_synth_vec(self._c, s.ctypes.data_as(_dp), t.ctypes.data_as(_dp), _p(vt), _p(vp))
+        return vt, vp
+
+    def anal_vec(self, vt, vp):
+        vt, vp = self._grid(vt), self._grid(vp)
+        s, t = self._s(), self._s()
+        lib().orc_anal_vec(self._c, _p(vt), _p(vp), s.ctypes.data_as(_dp), t.ctypes.data_as(_dp))
+        return s, t
+
+    def synth_grad(self, s):
+        s = self._spec(s)
+        gt, gp = self._g(), self._g()
+        lib().orc_synth_grad(self._c, s.ctypes.data_as(_dp), _p(gt), _p(gp))
+        return gt, gp
+
+    def laplacian(self, q, inverse=False):
+        q = self._spec(q).copy()
+        lib().orc_laplacian(self._c, q.ctypes.data_as(_dp), int(bool(inverse)))
+        return q
+
+    def surface_integral(self, g):
+        return lib().orc_surface_integral(self._c, _p(self._grid(g)))
+
+    def uv_from_vordiv(self, vor, div):
+        vor, div = self._spec(vor), self._spec(div)
+        u, v = self._g(), self._g()
+        lib().orc_uv_from_vordiv(self._c, vor.ctypes.data_as(_dp), div.ctypes.data_as(_dp), _p(u), _p(v))
+        return u, v
+
+    def vordiv_from_uv(self, u, v):
+        u, v = self._grid(u), self._grid(v)
+        vor, div = self._s(), self._s()
+        lib().orc_vordiv_from_uv(self._c, _p(u), _p(v), vor.ctypes.data_as(_dp), div.ctypes.data_as(_dp))
+        return vor, div
+
+    def gradient(self, f):
+        f = self._spec(f)
+        dx, dy = self._g(), self._g()
+        lib().orc_gradient(self._c, f.ctypes.data_as(_dp), _p(dx), _p(dy))
+        return dx, dy
+
+    def tendency_grid_transforms(self, vor, div, temp, lnps):
+        """Returns dict of grid fields; 3-D inputs are (nlev, nlm)."""
+        nlev = vor.shape[0]
+        vor, div, temp = (np.ascontiguousarray(a, dtype=np.complex128) for a in (vor, div, temp))
+        lnps = self._spec(lnps)
+        sh3 = (nlev, self.nlat, self.nlon)
+        out = {k: np.empty(sh3) for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy")}
+        out.update({k: self._g() for k in ("lnps_g", "dlnpsdx", "dlnpsdy")})
+        lib().orc_tendency_grid_transforms(
+            self._c, nlev, *[a.ctypes.data_as(_dp) for a in (vor, div, temp, lnps)],
+            *[_p(out[k]) for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
+                                   "lnps_g", "dlnpsdx", "dlnpsdy")])
+        return out
+
+    def tendency_spectral(self, ae, an, ek, dtdt, dlnpsdt):
+        nlev = ae.shape[0]
+        ae, an, ek, dtdt = (np.ascontiguousarray(a, dtype=np.float64) for a in (ae, an, ek, dtdt))
+        dlnpsdt = self._grid(dlnpsdt)
+        tv = np.empty((nlev, self.nlm), dtype=np.complex128)
+        td = np.empty_like(tv)
+        tt = np.empty_like(tv)
+        tl = self._s()
+        lib().orc_tendency_spectral(self._c, nlev, _p(ae), _p(an), _p(ek), _p(dtdt), _p(dlnpsdt),
+                                    tv.ctypes.data_as(_dp), td.ctypes.data_as(_dp),
+                                    tt.ctypes.data_as(_dp), tl.ctypes.data_as(_dp))
+        return {"vor": tv, "div": td, "temp": tt, "lnps": tl}
+
+    def bench_set(self, spec_in, nrep, nthreads):
+        """Seconds per (synthesis + analysis) unit over an OpenMP loop of units."""
+        spec_in = np.ascontiguousarray(spec_in, dtype=np.complex128)
+        nlev = spec_in.shape[0]
+        gw = np.empty((nlev, self.nlat, self.nlon))
+        so = np.empty_like(spec_in)
+        return lib().orc_bench_set(self._c, nlev, nrep, nthreads, spec_in.ctypes.data_as(_dp),
+                                   _p(gw), so.ctypes.data_as(_dp))

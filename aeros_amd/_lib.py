@@ -42,10 +42,13 @@ SIGNATURES = {
     "aer_sht_get_lm": (_c_int, [_vp, _ip, _ip]),
     "aer_sht_get_cos_theta": (_c_int, [_vp, _dp]),
     "aer_sht_gauss_wts": (_c_int, [_vp, _dp]),
+    "aer_sht_host_tables": (_c_int, [_c_int, _c_int, _dp, _dp, _dp, _dp, _dp]),
     "aer_sht_destroy": (_c_int, [_vp]),
     "aer_sht_set_precision": (_c_int, [_vp, _c_int]),
     "aer_sht_set_stream": (_c_int, [_vp, _vp]),
     "aer_sht_sync": (_c_int, [_vp]),
+    "aer_sht_timing": (_c_int, [_vp, _c_int]),
+    "aer_sht_timing_read": (_c_int, [_vp, _dp, _ip]),
     "aer_spat_to_SH": (_c_int, [_vp, _vp, _vp]),
     "aer_SH_to_spat": (_c_int, [_vp, _vp, _vp]),
     "aer_spat_to_SHsphtor": (_c_int, [_vp, _vp, _vp, _vp, _vp]),

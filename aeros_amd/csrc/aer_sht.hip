@@ -68,6 +68,11 @@ struct aer_sht {
     std::vector<int> radix, twoff;
     // device data
     DevBuf tab64[3], tab32[3], lfac, wpair, tw64, tw32, four, stage_s, stage_g;
+    // per-kernel event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int kind; hipEvent_t a, b; };
+    std::vector<Rec> recs;
 };
 
 namespace {
@@ -229,6 +234,38 @@ aer::CfgDev cfg_dev(const aer_sht *h, int prec)
 
 // ------------------------------------------------------------------ job launch
 
+hipEvent_t take_event(aer_sht *h)
+{
+    if (!h->ev_pool.empty()) {
+        hipEvent_t e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch one kernel, bracketed by events when timing is on.
+template <class F>
+int timed_launch(aer_sht *h, int kind, F &&launch)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (h->timing) {
+        a = take_event(h);
+        b = take_event(h);
+        if (!a || !b) return fail(AER_EHIP, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(a, h->stream));
+    }
+    launch();
+    HIP_TRY(hipGetLastError());
+    if (h->timing) {
+        HIP_TRY(hipEventRecord(b, h->stream));
+        h->recs.push_back({kind, a, b});
+    }
+    return AER_OK;
+}
+
 struct JobSpec {
     int kind, nitem;
     const double *sin0 = nullptr, *sin1 = nullptr;
@@ -296,18 +333,23 @@ int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
     aer::LaunchArgs LL = L, LF = L;
     LL.ntask = ltask;
     LF.ntask = ftask;
+    hipStream_t st = h->stream;
     if (synth) {
-        hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, h->stream, LL);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(aer::k_fft_synth<PREC>, dim3(ftask), dim3(256), lds, h->stream, LF);
-        HIP_TRY(hipGetLastError());
-    } else {
-        hipLaunchKernelGGL(aer::k_fft_anal<PREC>, dim3(ftask), dim3(256), lds, h->stream, LF);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, h->stream, LL);
-        HIP_TRY(hipGetLastError());
+        rc = timed_launch(h, 0, [&] {
+            hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, st, LL);
+        });
+        if (rc) return rc;
+        return timed_launch(h, 1, [&] {
+            hipLaunchKernelGGL(aer::k_fft_synth<PREC>, dim3(ftask), dim3(256), lds, st, LF);
+        });
     }
-    return AER_OK;
+    rc = timed_launch(h, 2, [&] {
+        hipLaunchKernelGGL(aer::k_fft_anal<PREC>, dim3(ftask), dim3(256), lds, st, LF);
+    });
+    if (rc) return rc;
+    return timed_launch(h, 3, [&] {
+        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, st, LL);
+    });
 }
 
 int run_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
@@ -484,6 +526,21 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     return AER_OK;
 }
 
+int aer_sht_host_tables(int lmax, int nlat, double *ct, double *w, double *P, double *D, double *Q)
+{
+    if (lmax < 0 || nlat < 1 || !ct || !w || !P || !D || !Q) return fail(AER_EINVAL, "aer_sht_host_tables: bad argument");
+    std::vector<double> x, ww, tP, tD, tQ;
+    gauss_legendre(nlat, x, ww);
+    const int nh = (nlat + 1) / 2, nlm = (lmax + 1) * (lmax + 2) / 2;
+    legendre_tables(lmax, nh, nh, x, tP, tD, tQ);
+    std::memcpy(ct, x.data(), sizeof(double) * nlat);
+    std::memcpy(w, ww.data(), sizeof(double) * nlat);
+    std::memcpy(P, tP.data(), sizeof(double) * (size_t)nlm * nh);
+    std::memcpy(D, tD.data(), sizeof(double) * (size_t)nlm * nh);
+    std::memcpy(Q, tQ.data(), sizeof(double) * (size_t)nlm * nh);
+    return AER_OK;
+}
+
 int aer_sht_get_info(aer_sht_t h, aer_sht_info *info)
 {
     if (!h || !info) return fail(AER_EINVAL, "aer_sht_get_info: null argument");
@@ -522,9 +579,43 @@ int aer_sht_gauss_wts(aer_sht_t h, double *wts_half)
     return AER_OK;
 }
 
+int aer_sht_timing(aer_sht_t h, int enable)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    for (auto &r : h->recs) {
+        h->ev_pool.push_back(r.a);
+        h->ev_pool.push_back(r.b);
+    }
+    h->recs.clear();
+    h->timing = enable != 0;
+    return AER_OK;
+}
+
+int aer_sht_timing_read(aer_sht_t h, double *total_ms, int *launches)
+{
+    if (!h || !total_ms || !launches) return fail(AER_EINVAL, "aer_sht_timing_read: null argument");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < AER_NKERNEL_KIND; ++k) {
+        total_ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    for (auto &r : h->recs) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        total_ms[r.kind] += ms;
+        launches[r.kind] += 1;
+    }
+    return AER_OK;
+}
+
 int aer_sht_destroy(aer_sht_t h)
 {
     if (!h) return AER_OK;
+    for (auto &r : h->recs) {
+        h->ev_pool.push_back(r.a);
+        h->ev_pool.push_back(r.b);
+    }
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     for (auto &b : h->tab64) b.release();
     for (auto &b : h->tab32) b.release();
     h->lfac.release();

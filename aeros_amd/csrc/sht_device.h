@@ -85,17 +85,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <int PREC> struct Prec;
+// Operand layouts of the 16x16x4 MFMAs (lane L): A[L%16][L/16], B[L/16][L%16] for
+// both types. Accumulator element r of lane L holds D[row(L/16, r)][L%16], where
+// row is 4r + L/16 for f64 and 4(L/16) + r for f32 (measured on gfx950).
 template <> struct Prec<0> {
     using T = double;
     using T2 = cx<double>;
     using Acc = d4;
     __device__ static inline Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static inline int row(int kq, int r) { return 4 * r + kq; }
 };
 template <> struct Prec<1> {
     using T = float;
     using T2 = cx<float>;
     using Acc = f4;
     __device__ static inline Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static inline int row(int kq, int r) { return 4 * kq + r; }
 };
 
 __device__ inline int find_job(const LaunchArgs &A, int task, bool legendre)
@@ -105,13 +110,12 @@ __device__ inline int find_job(const LaunchArgs &A, int task, bool legendre)
     return jb;
 }
 
-// 0-based index of (l = m, m) in the m-major, l-fastest layout.
-__device__ inline int lm_base(int T, int m) { return m * (2 * T + 1 - m) / 2; }
+// 0-based index of (l = m, m) in the m-major, l-fastest layout: the orders m' < m
+// hold sum (T+1-m') = m(2T+3-m)/2 coefficients (reference src/aeros_spectral.f90:478).
+__device__ inline int lm_base(int T, int m) { return m * (2 * T + 3 - m) / 2; }
 
 // ------------------------------------------------------------------ Legendre synthesis
 // One wavefront per task (job, group of 8 grid fields, m, tile of 16 latitude pairs).
-// MFMA 16x16x4 operand layout (lane L): A[L%16][L/16], B[L/16][L%16],
-// D[4*(L/16)+r][L%16] for r = 0..3.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
 {
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
     T *four = (T *)A.four + J.four_off;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int pair = pt * 16 + kq * 4 + r;
+        const int pair = pt * 16 + P::row(kq, r);
         const long base = (((long)g * (C.T + 1) + m) * C.nhp + pair) * 32;
         four[base + i] = E[r];
         four[base + 16 + i] = O[r];
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     const double *fe = C.lfac + J.mode2 * (C.T + 1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int kr = tile * 16 + kq * 4 + r;
+        const int kr = tile * 16 + P::row(kq, r);
         if (kr >= nk) continue;
         const int l = m + par + 2 * kr;
         const int lm = lmb + (l - m);

@@ -1,0 +1,303 @@
+"""Benchmark: spherical-harmonic transform sets per second at T42L20 on one MI355X.
+
+One step = one S_bl transform set of a dynamics step at T42 (128x64 Gaussian
+grid, lmax 42) with 20 levels, all fields resident in HBM:
+
+  synthesis half  lnps, grad lnps; per level (u,v) from (zeta,D), zeta, D, T, q,
+                  grad T                                   (aer_tendency_grid_transforms_d
+                                                            + aer_synth_d for q)
+  analysis half   per level (zeta_t, D_t) from (A_e, A_n) with D_t -= lap(Phi+K),
+                  T_t, q_t; lnps_t                         (aer_tendency_spectral_d
+                                                            + aer_anal_d for q)
+
+= 13L+4 = 264 scalar-equivalent transforms. `value` is the batched
+mixed-precision form (FP32 MFMA Legendre GEMM + FP32 FFT). Also measured on the
+same GPU: the batched FP64 form and the FP64 plain form (one call per level and
+field, the reference's loop structure), plus the per-field max relative error
+of the mixed form against FP64. Inputs are seeded synthetic states.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+T, NLEV = 42, 20
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+KIND_NAMES = ["k_leg_synth", "k_fft_synth", "k_fft_anal", "k_leg_anal"]
+
+
+def set_counts(L):
+    """Transform counts of S_bl: scalar, vector and gradient transforms, by direction."""
+    return {"syn_scalar": 4 * L + 1, "syn_vector": L, "syn_grad": L + 1,
+            "ana_scalar": 3 * L + 1, "ana_vector": L}
+
+
+def algorithmic(nlon, nlat, nlm, L):
+    """Algorithmic bytes (FP64 I/O) and dense flops per set, per kernel kind.
+
+    G = 8 nlon nlat, S = 16 nlm; Legendre flops per scalar contraction 4 nlat nlm;
+    a vector transform is 4 contractions, a gradient 2; FFT 2.5 nlon log2(nlon) per row."""
+    c = set_counts(L)
+    G, S = 8 * nlon * nlat, 16 * nlm
+    L1 = 4 * nlat * nlm
+    F1 = nlat * 2.5 * nlon * np.log2(nlon)
+    bytes_ = [S * (c["syn_scalar"] + 2 * c["syn_vector"] + c["syn_grad"]),
+              G * (c["syn_scalar"] + 2 * c["syn_vector"] + 2 * c["syn_grad"]),
+              G * (c["ana_scalar"] + 2 * c["ana_vector"]),
+              S * (c["ana_scalar"] + 2 * c["ana_vector"])]
+    flops = [L1 * (c["syn_scalar"] + 4 * c["syn_vector"] + 2 * c["syn_grad"]),
+             F1 * (c["syn_scalar"] + 2 * c["syn_vector"] + 2 * c["syn_grad"]),
+             F1 * (c["ana_scalar"] + 2 * c["ana_vector"]),
+             L1 * (c["ana_scalar"] + 4 * c["ana_vector"])]
+    return bytes_, flops
+
+
+class TransformSet:
+    """Device-resident inputs and outputs of one S_bl set, and the calls that run it."""
+
+    def __init__(self, sht, torch, seed_config=1):
+        from aeros_amd.seeding import red_spectra, red_spectrum
+        self.sht, self.torch = sht, torch
+        L, nlm = NLEV, sht.nlm
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        self.vor = dev(red_spectra(T, L, seed_config, field0=0, zero_l0=True) * 1e-5)
+        self.div = dev(red_spectra(T, L, seed_config, field0=100, zero_l0=True) * 1e-6)
+        self.temp = dev(red_spectra(T, L, seed_config, field0=200) * 5.0)
+        self.q = dev(red_spectra(T, L, seed_config, field0=300) * 1e-3)
+        self.lnps = dev(red_spectrum(T, 1000 * seed_config + 400) * 1e-2)
+        # grid inputs of the analysis half: nonlinear products of synthesized fields
+        sht.set_precision("fp64")
+        a = sht.synthesis(dev(red_spectra(T, L, seed_config, field0=500)))
+        b = sht.synthesis(dev(red_spectra(T, L, seed_config, field0=600)))
+        c = sht.synthesis(dev(red_spectra(T, L, seed_config, field0=700)))
+        torch.cuda.synchronize()
+        self.ae = (a * b * 1e-4).contiguous()
+        self.an = (b * c * 1e-4 - a * 1e-5).contiguous()
+        self.ek = (a * a * 50.0 + c * 100.0).contiguous()
+        self.dtdt = (b * c * 1e-5).contiguous()
+        self.qdot = (a * c * 1e-8).contiguous()
+        self.dlnpsdt = (a[0] * 1e-6 + b[1] * c[2] * 1e-6).contiguous()
+        g = lambda: torch.empty((L, sht.nlat, sht.nlon), dtype=torch.float64, device="cuda")
+        g2 = lambda: torch.empty((sht.nlat, sht.nlon), dtype=torch.float64, device="cuda")
+        s = lambda: torch.empty((L, nlm), dtype=torch.complex128, device="cuda")
+        self.gout = {k: g() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy")}
+        self.gout.update({k: g2() for k in ("lnps_g", "dlnpsdx", "dlnpsdy")})
+        self.q_g = g()
+        self.sout = {k: s() for k in ("vor", "div", "temp")}
+        self.sout["lnps"] = torch.empty((nlm,), dtype=torch.complex128, device="cuda")
+        self.q_t = s()
+
+    def run_batched(self):
+        from aeros_amd._lib import check, lib
+        sh, L = self.sht, NLEV
+        h, ss, gs = sh._h, 2 * sh.nlm, sh.nlon * sh.nlat
+        o, so = self.gout, self.sout
+        check(lib().aer_tendency_grid_transforms_d(
+            h, L, self.vor.data_ptr(), self.div.data_ptr(), self.temp.data_ptr(), self.lnps.data_ptr(),
+            *[o[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
+                                        "lnps_g", "dlnpsdx", "dlnpsdy")]))
+        check(lib().aer_synth_d(h, L, self.q.data_ptr(), ss, self.q_g.data_ptr(), gs))
+        check(lib().aer_tendency_spectral_d(
+            h, L, self.ae.data_ptr(), self.an.data_ptr(), self.ek.data_ptr(), self.dtdt.data_ptr(),
+            self.dlnpsdt.data_ptr(), so["vor"].data_ptr(), so["div"].data_ptr(), so["temp"].data_ptr(),
+            so["lnps"].data_ptr()))
+        check(lib().aer_anal_d(h, L, self.qdot.data_ptr(), gs, self.q_t.data_ptr(), ss))
+
+    def run_plain(self):
+        """The reference's loop structure: one call per level and per transform."""
+        from aeros_amd._lib import check, lib
+        sh, L = self.sht, NLEV
+        h, ss, gs = sh._h, 2 * sh.nlm, sh.nlon * sh.nlat
+        o, so = self.gout, self.sout
+        sb, gb = 16 * sh.nlm, 8 * gs   # bytes per spectral / grid field
+        P = lambda t, k, nb: t.data_ptr() + k * nb
+        check(lib().aer_synth_d(h, 1, self.lnps.data_ptr(), ss, o["lnps_g"].data_ptr(), gs))
+        check(lib().aer_gradient_d(h, 1, self.lnps.data_ptr(), o["dlnpsdx"].data_ptr(), o["dlnpsdy"].data_ptr()))
+        for k in range(L):
+            check(lib().aer_uv_from_vordiv_d(h, 1, P(self.vor, k, sb), P(self.div, k, sb),
+                                             P(o["u"], k, gb), P(o["v"], k, gb)))
+            check(lib().aer_synth_d(h, 1, P(self.vor, k, sb), ss, P(o["vor_g"], k, gb), gs))
+            check(lib().aer_synth_d(h, 1, P(self.div, k, sb), ss, P(o["div_g"], k, gb), gs))
+            check(lib().aer_synth_d(h, 1, P(self.temp, k, sb), ss, P(o["t_g"], k, gb), gs))
+            check(lib().aer_gradient_d(h, 1, P(self.temp, k, sb), P(o["dtdx"], k, gb), P(o["dtdy"], k, gb)))
+            check(lib().aer_synth_d(h, 1, P(self.q, k, sb), ss, P(self.q_g, k, gb), gs))
+        for k in range(L):
+            check(lib().aer_tendency_spectral_d(
+                h, 1, P(self.ae, k, gb), P(self.an, k, gb), P(self.ek, k, gb), P(self.dtdt, k, gb),
+                self.dlnpsdt.data_ptr(), P(so["vor"], k, sb), P(so["div"], k, sb), P(so["temp"], k, sb),
+                so["lnps"].data_ptr()))
+            check(lib().aer_anal_d(h, 1, P(self.qdot, k, gb), gs, P(self.q_t, k, sb), ss))
+
+    def outputs(self):
+        out = {k: v.clone() for k, v in self.gout.items()}
+        out["q_g"] = self.q_g.clone()
+        out.update({"tnd_" + k: v.clone() for k, v in self.sout.items()})
+        out["tnd_q"] = self.q_t.clone()
+        return out
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps   # ms per step
+
+
+def kernel_times(sht, fn, steps):
+    """Per-kernel-kind milliseconds per set and launches per set, from HIP events."""
+    from aeros_amd._lib import check, lib
+    check(lib().aer_sht_timing(sht._h, 1))
+    for _ in range(steps):
+        fn()
+    ms = (ctypes.c_double * 4)()
+    n = (ctypes.c_int * 4)()
+    check(lib().aer_sht_timing_read(sht._h, ms, n))
+    check(lib().aer_sht_timing(sht._h, 0))
+    return [ms[k] / steps for k in range(4)], [n[k] / steps for k in range(4)]
+
+
+def rel_err(a, ref):
+    a = a.cpu().numpy()
+    ref = ref.cpu().numpy()
+    if a.ndim >= 2 and a.shape[0] == NLEV:
+        return max(float(np.abs(a[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in range(NLEV))
+    return float(np.abs(a - ref).max() / np.abs(ref).max())
+
+
+def cpu_baseline(budget_s=12.0):
+    """The FP64 CPU restatement (oracle/) timed in the reference's mode-B harness shape:
+    an OpenMP loop over levels, each unit one scalar synthesis + one analysis."""
+    from aeros_amd.seeding import red_spectra
+    from oracle.oracle import Oracle
+    o = Oracle(T)
+    q = red_spectra(T, NLEV, config=1)
+    threads = min(16, os.cpu_count() or 1)
+    t_unit = o.bench_set(q, 1, threads)
+    nrep = max(1, min(200, int(budget_s / max(t_unit * NLEV, 1e-6))))
+    t_unit = o.bench_set(q, nrep, threads)
+    units_per_set = (13 * NLEV + 4) / 2.0   # 264 scalar-equivalents = 132 (synthesis + analysis) units
+    return {"value": 1.0 / (t_unit * units_per_set), "unit": "sets/s", "cores": threads,
+            "kind": "port",
+            "sample": f"T42 FP64 oracle (dense Gauss-Legendre + own mixed-radix FFT), {NLEV} levels x "
+                      f"{nrep} reps of scalar synthesis+analysis, OpenMP over levels; "
+                      f"{t_unit * 1e6 / 2:.1f} us per scalar-equivalent transform; sets/s = 1/(132 units)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the transform set of one member is not split across "
+              "GPUs (ensemble members shard with no exchange; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from aeros_amd.spectral import AerosSht
+    torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 8, "launches": 5}))
+        return
+    sht = AerosSht(T)
+    ts = TransformSet(sht, torch)
+    sht._bind_stream()
+
+    # FP64 reference outputs and the FP64 forms
+    sht.set_precision("fp64")
+    ts.run_batched()
+    ref = ts.outputs()
+    plain_steps = max(5, args.steps // 10)
+    # the per-level tendency call also re-analyses dlnps/dt on every level: 20 extra
+    # single-level transforms beyond the set's 264; normalize the plain time to 264
+    ms_plain = time_steps(torch, ts.run_plain, plain_steps, 2) * (13 * NLEV + 4) / (14 * NLEV + 3)
+    ms_fp64 = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    k64_ms, _ = kernel_times(sht, ts.run_batched, max(10, args.steps // 4))
+
+    sht.set_precision(args.precision)
+    ts.run_batched()
+    errs = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
+    ms = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    kms, klaunch = kernel_times(sht, ts.run_batched, max(10, args.steps // 4))
+
+    bytes_k, flops_k = algorithmic(sht.nlon, sht.nlat, sht.nlm, NLEV)
+    set_bytes = sum(bytes_k)
+    kernels = {}
+    for k in range(4):
+        hbm = bytes_k[k] / (kms[k] * 1e-3) / 1e9
+        tf = flops_k[k] / (kms[k] * 1e-3) / 1e12
+        kernels[KIND_NAMES[k]] = {"ms_per_set": round(kms[k], 5), "launches_per_set": klaunch[k],
+                                  "alg_bytes_per_set": int(bytes_k[k]), "alg_flops_per_set": int(flops_k[k]),
+                                  "hbm_GBs": round(hbm, 1), "TFLOPs": round(tf, 3),
+                                  "fp64_ms_per_set": round(k64_ms[k], 5)}
+    dom = int(np.argmax(kms))
+    pk = PEAK_TFLOPS[args.precision]
+    if dom in (0, 3):   # Legendre GEMM: MFMA-bound by its dense flop count
+        ach = flops_k[dom] / (kms[dom] * 1e-3) / 1e12
+        roof = {"kernel": KIND_NAMES[dom], "bound": "mfma", "achieved": round(ach, 3), "peak": pk,
+                "unit": "TFLOP/s", "frac": round(ach / pk, 5), "traffic": None}
+    else:
+        ach = bytes_k[dom] / (kms[dom] * 1e-3) / 1e9
+        roof = {"kernel": KIND_NAMES[dom], "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None}
+    roof["alg_per_launch"] = "see kernels[].alg_*_per_set / launches_per_set"
+
+    line = {
+        "metric": "SH transform sets/s at T42L20, mixed-prec vs FP64 on same GPU at stated rel err",
+        "value": round(1000.0 / ms, 2),
+        "unit": "sets/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded red-spectrum states, SplitMix64 + Box-Muller)",
+        "config": {"workload": "T42L20 S_bl transform set (13L+4 = 264 scalar-equivalent transforms: "
+                               "lnps, grad lnps; per level (u,v), zeta, D, T, q, grad T synthesis; "
+                               "(zeta_t, D_t - lap(Phi+K)), T_t, q_t, lnps_t analysis)",
+                   "trunc": T, "nlev": NLEV, "grid": f"{sht.nlon}x{sht.nlat}",
+                   "precision": "fp32-mfma" if args.precision == "fp32" else "fp64-mfma",
+                   "io": "fp64 in HBM"},
+        "fp64_batched_sets_per_s": round(1000.0 / ms_fp64, 2),
+        "fp64_plain_sets_per_s": round(1000.0 / ms_plain, 2),
+        "speedup_vs_fp64_plain": round(ms_plain / ms, 2),
+        "speedup_vs_fp64_batched": round(ms_fp64 / ms, 3),
+        "max_rel_err_vs_fp64": max(errs.values()),
+        "rel_err_vs_fp64_per_field": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "set_hbm_GBs": round(set_bytes / (ms * 1e-3) / 1e9, 1),
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -93,6 +93,12 @@ int aer_sht_get_lm(aer_sht_t cfg, int *li, int *mi);
 int aer_sht_get_cos_theta(aer_sht_t cfg, double *ct);
 /* Gauss weights of the northern half, nlat_2 values (shtns_gauss_wts, src/aeros_spectral.f90:271). */
 int aer_sht_gauss_wts(aer_sht_t cfg, double *wts_half);
+/* Host-only geometry and FP64 Legendre tables, exactly as set_grid uploads them:
+ * ct[nlat] (cos colatitude, N->S), w[nlat] (Gauss weights), and P, dP/dtheta,
+ * m P/sin(theta) at the northern nodes, each [nlm][(nlat+1)/2]. Needs no GPU; no
+ * reference counterpart (SHTns keeps its tables internal, src/aeros_spectral.f90:246-275
+ * reads back only ct and the weights). */
+int aer_sht_host_tables(int lmax, int nlat, double *ct, double *w, double *P, double *D, double *Q);
 /* Replaces shtns_destroy (src/aeros_spectral.f90:288). Accepts NULL. */
 int aer_sht_destroy(aer_sht_t cfg);
 
@@ -156,6 +162,16 @@ int aer_tendency_spectral_d(aer_sht_t cfg, int nlev,
 
 /* Block until all work queued on the handle's stream has finished. */
 int aer_sht_sync(aer_sht_t cfg);
+
+/* ---- observability: per-kernel timing with HIP events recorded on the handle's
+ * stream around every kernel launch (off by default; no reference counterpart --
+ * the reference times whole transforms with system_clock, drivers/bench_sht.f90:214-227).
+ * Kernel kinds: 0 Legendre synthesis, 1 FFT synthesis, 2 FFT analysis, 3 Legendre analysis.
+ * aer_sht_timing(cfg, 1) starts and clears; aer_sht_timing_read() synchronizes the stream
+ * and returns the summed milliseconds and launch counts per kind since the start. */
+#define AER_NKERNEL_KIND 4
+int aer_sht_timing(aer_sht_t cfg, int enable);
+int aer_sht_timing_read(aer_sht_t cfg, double *total_ms, int *launches);
 
 #ifdef __cplusplus
 }

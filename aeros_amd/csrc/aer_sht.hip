@@ -58,7 +58,7 @@ struct DevBuf {
 
 struct aer_sht {
     int lmax = 0, mmax = 0, mres = 1, norm = 0, layout = 0;
-    int nlat = 0, nlon = 0, nlm = 0, nh = 0, nhp = 0, npt = 0, pb = 1, fb = 8;
+    int nlat = 0, nlon = 0, nlm = 0, nh = 0, nhp = 0, npt = 0, pb = 1, fb = 8, kp = 16;
     double eps = 0.0;
     bool grid_set = false;
     int prec = AER_PREC_FP64;
@@ -67,7 +67,8 @@ struct aer_sht {
     std::vector<int> li, mi;
     std::vector<int> radix, twoff;
     // device data
-    DevBuf tab64[3], tab32[3], lfac, wpair, tw64, tw32, four, stage_s, stage_g;
+    DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
+    std::vector<double> tw_host, twN_host;  // per-pass twiddles; e^{-2 pi i k/nlon}
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -161,6 +162,23 @@ void legendre_tables(int T, int nh, int nhp, const std::vector<double> &ct,
     }
 }
 
+// Analysis copy of a [lm][nhp] table: [m][parity][nhp][kp] with l = m + parity + 2k
+// fastest, zero where l > T.
+std::vector<double> transpose_table(const std::vector<double> &X, int T, int nhp, int kp)
+{
+    std::vector<double> Y((size_t)(T + 1) * 2 * nhp * kp, 0.0);
+    for (int m = 0; m <= T; ++m) {
+        const int lmb = m * (2 * T + 3 - m) / 2;
+        for (int par = 0; par < 2; ++par)
+            for (int k = 0; m + par + 2 * k <= T; ++k) {
+                const int lm = lmb + par + 2 * k;
+                for (int j = 0; j < nhp; ++j)
+                    Y[(((size_t)m * 2 + par) * nhp + j) * kp + k] = X[(size_t)lm * nhp + j];
+            }
+    }
+    return Y;
+}
+
 int upload(DevBuf &b, const void *src, size_t bytes)
 {
     int rc = b.ensure(bytes);
@@ -197,12 +215,15 @@ int ensure_tables32(aer_sht *h)
         auto f = convert<float>(*src[k]);
         int rc = upload(h->tab32[k], f.data(), f.size() * sizeof(float));
         if (rc) return rc;
+        auto ft = convert<float>(transpose_table(*src[k], h->lmax, h->nhp, h->kp));
+        rc = upload(h->tabT32[k], ft.data(), ft.size() * sizeof(float));
+        if (rc) return rc;
     }
-    // twiddles
-    std::vector<double> tw64(h->tw64.bytes / sizeof(double));
-    HIP_TRY(hipMemcpy(tw64.data(), h->tw64.p, h->tw64.bytes, hipMemcpyDeviceToHost));
-    auto tw32 = convert<float>(tw64);
-    return upload(h->tw32, tw32.data(), tw32.size() * sizeof(float));
+    auto tw32 = convert<float>(h->tw_host);
+    int rc = upload(h->tw32, tw32.data(), tw32.size() * sizeof(float));
+    if (rc) return rc;
+    auto twN32 = convert<float>(h->twN_host);
+    return upload(h->twN32, twN32.data(), twN32.size() * sizeof(float));
 }
 
 aer::CfgDev cfg_dev(const aer_sht *h, int prec)
@@ -221,9 +242,15 @@ aer::CfgDev cfg_dev(const aer_sht *h, int prec)
     c.tabP = tab[0].p;
     c.tabD = tab[1].p;
     c.tabQ = tab[2].p;
+    const DevBuf *tabT = (prec == AER_PREC_FP32) ? h->tabT32 : h->tabT64;
+    c.tabPT = tabT[0].p;
+    c.tabDT = tabT[1].p;
+    c.tabQT = tabT[2].p;
+    c.kp = h->kp;
     c.lfac = (const double *)h->lfac.p;
     c.wpair = (const double *)h->wpair.p;
     c.tw = (prec == AER_PREC_FP32) ? h->tw32.p : h->tw64.p;
+    c.twN = (prec == AER_PREC_FP32) ? h->twN32.p : h->twN64.p;
     c.npass = (int)h->radix.size();
     for (int p = 0; p < c.npass; ++p) {
         c.radix[p] = h->radix[p];
@@ -278,21 +305,31 @@ struct JobSpec {
     int accumulate = 0;
 };
 
-template <int PREC>
-int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
-{
-    using T = typename aer::Prec<PREC>::T;
-    aer::LaunchArgs L{};
-    L.njob = 0;
+// Job table of one direction: jobs on handles hs[k] (one truncation each), their
+// task ranges in the Legendre and FFT launches, the intermediate size and the FFT
+// kernel's dynamic LDS.
+struct Table {
+    std::vector<aer::Job> jobs;
     int ltask = 0, ftask = 0;
     long four_elems = 0;
-    const aer::CfgDev C = cfg_dev(h, PREC);
-    const int ntpp = ((C.T + 2) / 2 + 15) / 16;
-    const int nsub = 8 / h->fb;
+    size_t lds = 0;
+    bool generic = false;   // some job needs the runtime-plan FFT path
+    int kmax = 1;           // max T+1 over the jobs
+};
+
+template <int PREC>
+void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
+{
+    using T = typename aer::Prec<PREC>::T;
+    B = Table{};
     for (int k = 0; k < njob; ++k) {
         const JobSpec &s = specs[k];
         if (s.nitem <= 0) continue;
-        aer::Job &J = L.job[L.njob++];
+        const aer_sht *h = hs[k];
+        const aer::CfgDev C = cfg_dev(h, PREC);
+        const int ntpp = ((C.T + 2) / 2 + 15) / 16;
+        const int nsub = 8 / h->fb;
+        aer::Job J{};
         J.c = C;
         J.kind = s.kind;
         J.nitem = s.nitem;
@@ -313,43 +350,85 @@ int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
         J.mode0 = s.mode0; J.mode1 = s.mode1; J.mode2 = s.mode2;
         J.gsc0 = s.gsc0; J.gsc1 = s.gsc1;
         J.accumulate = s.accumulate;
-        J.four_off = four_elems;
-        four_elems += (long)J.ngroup * (C.T + 1) * C.nhp * 32;
-        J.ltask0 = ltask;
-        J.ftask0 = ftask;
+        J.four_off = B.four_elems;
+        B.four_elems += (long)J.ngroup * (C.T + 1) * C.nhp * 32;
+        J.ltask0 = B.ltask;
+        J.ftask0 = B.ftask;
         if (synth) {
-            ltask += J.lgroups * (C.T + 1) * C.npt;
-            ftask += J.ngroup * nsub * ((C.nh + C.pb - 1) / C.pb);
+            B.ltask += J.lgroups * (C.T + 1) * C.npt;
+            B.ftask += J.ngroup * nsub * ((C.nh + C.pb - 1) / C.pb);
         } else {
-            ltask += J.lgroups * (C.T + 1) * 2 * ntpp;
-            ftask += J.ngroup * nsub * ((C.nhp + C.pb - 1) / C.pb);
+            B.ltask += J.lgroups * (C.T + 1) * 2 * ntpp;
+            B.ftask += J.ngroup * nsub * ((C.nhp + C.pb - 1) / C.pb);
         }
+        const size_t cxb = 2 * sizeof(T);
+        const size_t lds = aer::fft_fixed(h->nlon)
+                               ? ((size_t)2 * aer::fft_fixed_rows(h->nlon) + 1) * h->nlon * cxb
+                               : (size_t)2 * h->fb * h->pb * h->nlon * cxb;
+        B.lds = std::max(B.lds, lds);
+        B.generic = B.generic || !aer::fft_fixed(h->nlon);
+        B.kmax = std::max(B.kmax, C.T + 1);
+        B.jobs.push_back(J);
     }
-    if (L.njob == 0) return AER_OK;
-    int rc = h->four.ensure((size_t)four_elems * sizeof(T));
-    if (rc) return rc;
-    L.four = h->four.p;
-    const size_t lds = (size_t)2 * h->fb * h->pb * h->nlon * 2 * sizeof(T);
+}
+
+// The two launches of one direction. th: the handle whose timing hook records them.
+template <int PREC>
+int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
+                 const int *tj_leg = nullptr, const int *tj_fft = nullptr)
+{
+    using T = typename aer::Prec<PREC>::T;
     aer::LaunchArgs LL = L, LF = L;
-    LL.ntask = ltask;
-    LF.ntask = ftask;
-    hipStream_t st = h->stream;
+    LL.task_job = tj_leg;
+    LF.task_job = tj_fft;
+    LL.kmax = LF.kmax = B.kmax;
+    const size_t lds_leg = synth ? (size_t)4 * 8 * B.kmax * 2 * sizeof(T) : 0;
+    LL.ntask = B.ltask;
+    LF.ntask = B.ftask;
+    const int lb = (B.ltask + 3) / 4, fb = B.ftask;
+    const size_t lds = B.lds;
+    int rc;
     if (synth) {
-        rc = timed_launch(h, 0, [&] {
-            hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, st, LL);
+        rc = timed_launch(th, 0, [&] {
+            hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3(lb), dim3(256), lds_leg, st, LL);
         });
         if (rc) return rc;
-        return timed_launch(h, 1, [&] {
-            hipLaunchKernelGGL(aer::k_fft_synth<PREC>, dim3(ftask), dim3(256), lds, st, LF);
+        return timed_launch(th, 1, [&] {
+            if (B.generic)
+                hipLaunchKernelGGL((aer::k_fft_synth<PREC, true>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+            else
+                hipLaunchKernelGGL((aer::k_fft_synth<PREC, false>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
         });
     }
-    rc = timed_launch(h, 2, [&] {
-        hipLaunchKernelGGL(aer::k_fft_anal<PREC>, dim3(ftask), dim3(256), lds, st, LF);
+    rc = timed_launch(th, 2, [&] {
+        if (B.generic)
+            hipLaunchKernelGGL((aer::k_fft_anal<PREC, true>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+        else
+            hipLaunchKernelGGL((aer::k_fft_anal<PREC, false>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
     });
     if (rc) return rc;
-    return timed_launch(h, 3, [&] {
-        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3((ltask + 3) / 4), dim3(256), 0, st, LL);
+    return timed_launch(th, 3, [&] {
+        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3(lb), dim3(256), 0, st, LL);
     });
+}
+
+template <int PREC>
+int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
+{
+    using T = typename aer::Prec<PREC>::T;
+    aer_sht *hs[AER_MAXJOB];
+    for (int k = 0; k < njob; ++k) hs[k] = h;
+    Table B;
+    build_table<PREC>(hs, specs, njob, synth, B);
+    if (B.jobs.empty()) return AER_OK;
+    int rc = h->four.ensure((size_t)B.four_elems * sizeof(T));
+    if (rc) return rc;
+    aer::LaunchArgs L{};
+    L.njob = (int)B.jobs.size();
+    L.four = h->four.p;
+    L.djobs = nullptr;
+    for (int k = 0; k < L.njob; ++k) L.job[k] = B.jobs[k];
+    return launch_table<PREC>(h, h->stream, L, B, synth);
 }
 
 int run_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
@@ -464,17 +543,28 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     h->nh = (nlat + 1) / 2;
     h->nhp = (h->nh + 15) / 16 * 16;
     h->npt = h->nhp / 16;
-    // FFT block: fb grid fields x pb pairs, at most 2048 complex per LDS buffer
-    h->fb = 8;
-    h->pb = std::max(1, 256 / nphi);
-    while (h->fb * h->pb * nphi > 2048 && h->fb > 1) h->fb /= 2;
+    // FFT block: fb grid fields x pb latitude pairs. Lengths with a compile-time
+    // path use 8 fields x fft_fixed_rows/8 pairs; others at most 2048 complex per buffer.
+    if (aer::fft_fixed(nphi)) {
+        h->fb = 8;
+        h->pb = aer::fft_fixed_rows(nphi) / 8;
+    } else {
+        h->fb = 8;
+        h->pb = std::max(1, 128 / nphi);
+        while (h->fb * h->pb * nphi > 2048 && h->fb > 1) h->fb /= 2;
+    }
     gauss_legendre(nlat, h->ct, h->w);
     std::vector<double> P, D, Q;
     legendre_tables(h->lmax, h->nh, h->nhp, h->ct, P, D, Q);
     const std::vector<double> *tabs[3] = {&P, &D, &Q};
-    for (int k = 0; k < 3; ++k)
+    h->kp = (h->lmax / 2 + 1 + 15) / 16 * 16;
+    for (int k = 0; k < 3; ++k) {
         if ((rc = upload(h->tab64[k], tabs[k]->data(), tabs[k]->size() * sizeof(double)))) return rc;
+        const std::vector<double> tt = transpose_table(*tabs[k], h->lmax, h->nhp, h->kp);
+        if ((rc = upload(h->tabT64[k], tt.data(), tt.size() * sizeof(double)))) return rc;
+    }
     for (auto &b : h->tab32) b.release();
+    for (auto &b : h->tabT32) b.release();
     // per-degree factors
     const int T = h->lmax;
     std::vector<double> lf((size_t)AER_NMODE * (T + 1), 0.0);
@@ -521,7 +611,17 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     }
     if (tw.empty()) tw.assign(2, 0.0);
     if ((rc = upload(h->tw64, tw.data(), tw.size() * sizeof(double)))) return rc;
+    h->tw_host = tw;
+    std::vector<double> twN(2 * (size_t)nphi);
+    for (int k = 0; k < nphi; ++k) {
+        const double a = -2.0 * kPi * k / nphi;
+        twN[2 * k] = std::cos(a);
+        twN[2 * k + 1] = std::sin(a);
+    }
+    if ((rc = upload(h->twN64, twN.data(), twN.size() * sizeof(double)))) return rc;
+    h->twN_host = twN;
     h->tw32.release();
+    h->twN32.release();
     h->grid_set = true;
     return AER_OK;
 }
@@ -618,10 +718,14 @@ int aer_sht_destroy(aer_sht_t h)
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     for (auto &b : h->tab64) b.release();
     for (auto &b : h->tab32) b.release();
+    for (auto &b : h->tabT64) b.release();
+    for (auto &b : h->tabT32) b.release();
     h->lfac.release();
     h->wpair.release();
     h->tw64.release();
     h->tw32.release();
+    h->twN64.release();
+    h->twN32.release();
     h->four.release();
     h->stage_s.release();
     h->stage_g.release();
@@ -642,6 +746,220 @@ int aer_sht_set_stream(aer_sht_t h, void *stream)
 {
     if (!h) return fail(AER_EINVAL, "null handle");
     h->stream = (hipStream_t)stream;
+    return AER_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Public job descriptor -> internal job spec, with validation.
+int job_to_spec(const aer_sht *h, const aer_job &j, JobSpec &s)
+{
+    if (j.kind < AER_JOB_SYNTH_SCALAR || j.kind > AER_JOB_ANAL_VECTOR)
+        return fail(AER_EINVAL, "job: unknown kind");
+    const bool synth = j.kind <= AER_JOB_SYNTH_GRAD;
+    if (j.nfield < 0) return fail(AER_EINVAL, "job: nfield < 0");
+    for (int md : {j.mode0, j.mode1, j.mode2})
+        if (md < 0 || md >= AER_NMODE) return fail(AER_EINVAL, "job: unknown mode");
+    const bool vec_in = j.kind == AER_JOB_SYNTH_VECTOR || j.kind == AER_JOB_ANAL_VECTOR;
+    const bool two_out = j.kind == AER_JOB_SYNTH_VECTOR || j.kind == AER_JOB_SYNTH_GRAD || j.kind == AER_JOB_ANAL_VECTOR;
+    if (j.nfield > 0 && (!j.in0 || !j.out0 || (vec_in && !j.in1) || (two_out && !j.out1)))
+        return fail(AER_EINVAL, "job: missing array");
+    const long ss = dense_s(h), gs = dense_g(h);
+    s = JobSpec{j.kind, j.nfield};
+    const long is = j.in_stride ? j.in_stride : (synth ? ss : gs);
+    const long os = j.out_stride ? j.out_stride : (synth ? gs : ss);
+    if (is < (synth ? ss : gs) || os < (synth ? gs : ss))
+        return fail(AER_EINVAL, "job: stride smaller than a dense field");
+    if (synth) {
+        s.sin0 = j.in0; s.sin1 = j.in1; s.sstride = is;
+        s.gout0 = j.out0; s.gout1 = j.out1; s.gstride = os;
+    } else {
+        s.gin0 = j.in0; s.gin1 = j.in1; s.gin2 = j.kind == AER_JOB_ANAL_VECTOR ? j.in2 : nullptr;
+        s.gstride = is;
+        s.sout0 = j.out0; s.sout1 = j.out1; s.sstride = os;
+    }
+    s.mode0 = j.mode0; s.mode1 = j.mode1; s.mode2 = j.mode2;
+    s.gsc0 = j.scale0 != 0.0 ? j.scale0 : 1.0;
+    s.gsc1 = j.scale1 != 0.0 ? j.scale1 : 1.0;
+    s.accumulate = j.accumulate;
+    return AER_OK;
+}
+
+}  // namespace
+
+struct aer_plan {
+    int prec = AER_PREC_FP64;
+    aer_sht *th = nullptr;           // timing hook owner (first job's handle)
+    Table syn, ana;
+    DevBuf djob_syn, djob_ana, four;
+    DevBuf tj[4];                    // task -> job: synthesis Legendre, FFT; analysis FFT, Legendre
+};
+
+namespace {
+
+template <int PREC>
+int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &specs)
+{
+    using T = typename aer::Prec<PREC>::T;
+    std::vector<aer_sht *> hsy, han;
+    std::vector<JobSpec> ssy, san;
+    for (size_t k = 0; k < specs.size(); ++k) {
+        const bool synth = specs[k].kind <= aer::SYN_GRAD;
+        (synth ? hsy : han).push_back(hs[k]);
+        (synth ? ssy : san).push_back(specs[k]);
+    }
+    build_table<PREC>(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
+    build_table<PREC>(han.data(), san.data(), (int)san.size(), false, p->ana);
+    int rc;
+    if (!p->syn.jobs.empty() && (rc = upload(p->djob_syn, p->syn.jobs.data(), p->syn.jobs.size() * sizeof(aer::Job))))
+        return rc;
+    if (!p->ana.jobs.empty() && (rc = upload(p->djob_ana, p->ana.jobs.data(), p->ana.jobs.size() * sizeof(aer::Job))))
+        return rc;
+    // task -> job index tables of the four launches
+    const Table *tabs[2] = {&p->syn, &p->ana};
+    for (int d = 0; d < 2; ++d) {
+        const Table &B = *tabs[d];
+        if (B.jobs.empty()) continue;
+        std::vector<int> tl(B.ltask), tf(B.ftask);
+        for (size_t k = 0; k < B.jobs.size(); ++k) {
+            const int l1 = k + 1 < B.jobs.size() ? B.jobs[k + 1].ltask0 : B.ltask;
+            const int f1 = k + 1 < B.jobs.size() ? B.jobs[k + 1].ftask0 : B.ftask;
+            for (int t = B.jobs[k].ltask0; t < l1; ++t) tl[t] = (int)k;
+            for (int t = B.jobs[k].ftask0; t < f1; ++t) tf[t] = (int)k;
+        }
+        DevBuf &bl = p->tj[d == 0 ? 0 : 3], &bf = p->tj[d == 0 ? 1 : 2];
+        if ((rc = upload(bl, tl.data(), std::max<size_t>(tl.size(), 1) * sizeof(int)))) return rc;
+        if ((rc = upload(bf, tf.data(), std::max<size_t>(tf.size(), 1) * sizeof(int)))) return rc;
+    }
+    const long four = std::max(p->syn.four_elems, p->ana.four_elems);
+    return p->four.ensure((size_t)std::max(four, 1L) * sizeof(T));
+}
+
+template <int PREC>
+int plan_launch(aer_plan *p, hipStream_t st)
+{
+    aer::LaunchArgs L{};
+    L.four = p->four.p;
+    int rc;
+    if (!p->syn.jobs.empty()) {
+        L.njob = (int)p->syn.jobs.size();
+        L.djobs = (const aer::Job *)p->djob_syn.p;
+        if ((rc = launch_table<PREC>(p->th, st, L, p->syn, true, (const int *)p->tj[0].p, (const int *)p->tj[1].p)))
+            return rc;
+    }
+    if (!p->ana.jobs.empty()) {
+        L.njob = (int)p->ana.jobs.size();
+        L.djobs = (const aer::Job *)p->djob_ana.p;
+        if ((rc = launch_table<PREC>(p->th, st, L, p->ana, false, (const int *)p->tj[3].p, (const int *)p->tj[2].p)))
+            return rc;
+    }
+    return AER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_plan_t *plan)
+{
+    if (!plan) return fail(AER_EINVAL, "aer_plan_create: null output");
+    *plan = nullptr;
+    if (njob <= 0 || !cfgs || !jobs) return fail(AER_EINVAL, "aer_plan_create: empty job list");
+    std::vector<aer_sht *> hs(njob);
+    std::vector<JobSpec> specs(njob);
+    int prec = -1, rc;
+    for (int k = 0; k < njob; ++k) {
+        aer_sht *h = cfgs[k];
+        if (!h) return fail(AER_EINVAL, "aer_plan_create: null handle");
+        if (!h->grid_set) return fail(AER_ESTATE, "aer_plan_create: handle without a grid");
+        if (prec < 0) prec = h->prec;
+        if (h->prec != prec) return fail(AER_EINVAL, "aer_plan_create: handles differ in precision");
+        if ((rc = job_to_spec(h, jobs[k], specs[k]))) return rc;
+        if (h->prec == AER_PREC_FP32 && (rc = ensure_tables32(h))) return rc;
+        hs[k] = h;
+    }
+    aer_plan *p = new aer_plan();
+    p->prec = prec;
+    p->th = hs[0];
+    switch (prec) {
+        case AER_PREC_FP64: rc = plan_build<0>(p, hs, specs); break;
+        case AER_PREC_FP32: rc = plan_build<1>(p, hs, specs); break;
+        default: rc = fail(AER_EINVAL, "aer_plan_create: precision not available");
+    }
+    if (rc) {
+        aer_plan_destroy(p);
+        return rc;
+    }
+    *plan = p;
+    return AER_OK;
+}
+
+int aer_plan_run(aer_plan_t p, void *stream)
+{
+    if (!p) return fail(AER_EINVAL, "aer_plan_run: null plan");
+    switch (p->prec) {
+        case AER_PREC_FP64: return plan_launch<0>(p, (hipStream_t)stream);
+        case AER_PREC_FP32: return plan_launch<1>(p, (hipStream_t)stream);
+        default: return fail(AER_EINVAL, "aer_plan_run: precision not available");
+    }
+}
+
+int aer_plan_destroy(aer_plan_t p)
+{
+    if (!p) return AER_OK;
+    p->djob_syn.release();
+    p->djob_ana.release();
+    p->four.release();
+    for (auto &b : p->tj) b.release();
+    delete p;
+    return AER_OK;
+}
+
+int aer_run_jobs_d(aer_sht_t h, int njob, const aer_job *jobs)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (njob < 0 || njob > AER_MAX_JOBS || (njob > 0 && !jobs)) return fail(AER_EINVAL, "aer_run_jobs_d: bad job count");
+    if (njob == 0) return AER_OK;
+    static_assert(AER_MAX_JOBS <= AER_MAXJOB, "job table size");
+    JobSpec spec[AER_MAX_JOBS];
+    const bool synth = jobs[0].kind <= AER_JOB_SYNTH_GRAD;
+    for (int k = 0; k < njob; ++k) {
+        int rc = job_to_spec(h, jobs[k], spec[k]);
+        if (rc) return rc;
+        if ((jobs[k].kind <= AER_JOB_SYNTH_GRAD) != synth)
+            return fail(AER_EINVAL, "aer_run_jobs_d: synthesis and analysis jobs cannot share a launch");
+    }
+    return run_jobs(h, spec, njob, synth);
+}
+
+int aer_dev_malloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return fail(AER_EINVAL, "aer_dev_malloc: null output");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(AER_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return AER_OK;
+}
+
+int aer_dev_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return AER_OK;
+}
+
+int aer_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes && (!dst || !src)) return fail(AER_EINVAL, "aer_memcpy_h2d: null pointer");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return AER_OK;
+}
+
+int aer_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes && (!dst || !src)) return fail(AER_EINVAL, "aer_memcpy_d2h: null pointer");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return AER_OK;
 }
 

@@ -27,6 +27,7 @@
 #define AER_MAXJOB 8
 #define AER_MAXPASS 12
 #define AER_NMODE 8
+#define AER_FFT_THREADS 128
 
 namespace aer {
 
@@ -47,10 +48,13 @@ enum Mode {
 struct CfgDev {
     int T, nlat, nlon, nh, nhp, npt, nlm;  // nh latitude pairs, nhp padded to 16, npt = nhp/16
     int pb, fb;                            // FFT block: fb grid fields (divides 8) x pb latitude pairs
-    const void *tabP, *tabD, *tabQ;            // [nlm][nhp] in the compute type
-    const double *lfac;                        // [AER_NMODE][T+1]
-    const double *wpair;                       // [nhp] w_j * 2pi/nlon (x1/2 on a self-mirrored equator row)
-    const void *tw;                            // forward twiddles, complex, compute type
+    const void *tabP, *tabD, *tabQ;        // [nlm][nhp] in the compute type
+    const void *tabPT, *tabDT, *tabQT;     // [m][parity][nhp][kp]: degree l = m + parity + 2k fastest
+    int kp;                                // k extent of the transposed tables (multiple of 16)
+    const double *lfac;                    // [AER_NMODE][T+1]
+    const double *wpair;                   // [nhp] w_j * 2pi/nlon (x1/2 on a self-mirrored equator row)
+    const void *tw;                        // per-pass forward twiddles (generic FFT path)
+    const void *twN;                       // e^{-2 pi i k/nlon}, k < nlon (compile-time FFT paths)
     int npass;
     int radix[AER_MAXPASS];
     int twoff[AER_MAXPASS];
@@ -71,9 +75,14 @@ struct Job {
     long four_off;                    // element offset of the job's intermediate region
 };
 
+// Job table of one launch: inline in the kernel arguments (up to AER_MAXJOB jobs),
+// or in device memory (djobs != nullptr, any number of jobs; compiled plans).
 struct LaunchArgs {
     int njob, ntask;
+    int kmax;               // max T+1 over the jobs (LDS staging of the Legendre synthesis)
     void *four;
+    const Job *djobs;       // device job table (compiled plans)
+    const int *task_job;    // device task -> job index (compiled plans)
     Job job[AER_MAXJOB];
 };
 
@@ -84,10 +93,10 @@ template <class T> struct __attribute__((aligned(2 * sizeof(T)))) cx {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <int PREC> struct Prec;
 // Operand layouts of the 16x16x4 MFMAs (lane L): A[L%16][L/16], B[L/16][L%16] for
 // both types. Accumulator element r of lane L holds D[row(L/16, r)][L%16], where
 // row is 4r + L/16 for f64 and 4(L/16) + r for f32 (measured on gfx950).
+template <int PREC> struct Prec;
 template <> struct Prec<0> {
     using T = double;
     using T2 = cx<double>;
@@ -103,81 +112,145 @@ template <> struct Prec<1> {
     __device__ static inline int row(int kq, int r) { return 4 * kq + r; }
 };
 
-__device__ inline int find_job(const LaunchArgs &A, int task, bool legendre)
+// The job a task belongs to: the last job whose first task is <= task. The task
+// index is uniform over a wave, so the job index is too (readfirstlane lets the
+// compiler read the job's fields with scalar loads).
+__device__ __forceinline__ const Job &job_of(const LaunchArgs &A, int task, bool legendre)
 {
+    if (A.djobs) {
+        const int jb = __builtin_amdgcn_readfirstlane(A.task_job[task]);
+        return A.djobs[jb];
+    }
     int jb = 0;
     while (jb + 1 < A.njob && task >= (legendre ? A.job[jb + 1].ltask0 : A.job[jb + 1].ftask0)) ++jb;
-    return jb;
+    return A.job[jb];
 }
 
 // 0-based index of (l = m, m) in the m-major, l-fastest layout: the orders m' < m
 // hold sum (T+1-m') = m(2T+3-m)/2 coefficients (reference src/aeros_spectral.f90:478).
 __device__ inline int lm_base(int T, int m) { return m * (2 * T + 3 - m) / 2; }
 
+// MFMA steps are processed in chunks: the operands of a whole chunk are loaded
+// before its MFMAs issue, so one load latency covers the chunk. KCH_S steps of 4
+// degrees cover both parities up to T = 47 in one chunk; KCH_A steps of 4
+// latitude pairs cover 32 pairs (T42) in one chunk.
+constexpr int KCH_S = 12;
+constexpr int KCH_A = 8;
+constexpr int KCH = KCH_A;
+
 // ------------------------------------------------------------------ Legendre synthesis
 // One wavefront per task (job, group of 8 grid fields, m, tile of 16 latitude pairs).
+// The wave first stages the coefficients of order m of its fields in LDS (contiguous
+// in memory, so the loads coalesce), scaled by the job's per-degree factor; the MFMA
+// B operands are then read from LDS. Segment s of a wave holds field g*8+s (scalar
+// jobs) or S (s < 4) / T (s >= 4) of item g*4 + s%4 (vector and gradient jobs).
 template <int PREC>
 __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
 {
     using P = Prec<PREC>;
     using T = typename P::T;
     using Acc = typename P::Acc;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    extern __shared__ char smem_raw[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int task = blockIdx.x * 4 + wave;
-    if (task >= A.ntask) return;
-    const Job &J = A.job[find_job(A, task, true)];
+    const bool active = task < A.ntask;
+    cx<T> *sb = (cx<T> *)smem_raw + (long)wave * 8 * A.kmax;
+    const Job &J = job_of(A, active ? task : 0, true);
     const CfgDev &C = J.c;
     int t = task - J.ltask0;
     const int g = t % J.lgroups;
     t /= J.lgroups;
     const int pt = t % C.npt;
     const int m = t / C.npt;
-    const int i = lane & 15, kq = lane >> 4;
-    const int gf = g * 8 + (i >> 1), part = i & 1;
-    const bool cv = gf < J.ngf;
-    int item, comp;
-    if (J.kind == SYN_SCALAR) { item = gf; comp = 0; }
-    else { item = gf >> 1; comp = gf & 1; }
-    const double *s0 = J.sin0 + (long)item * J.sstride;
-    const double *s1 = (J.kind == SYN_VECTOR) ? J.sin1 + (long)item * J.sstride : nullptr;
     const int lmb = lm_base(C.T, m);
-    const T *tP = (const T *)C.tabP, *tD = (const T *)C.tabD, *tQ = (const T *)C.tabQ;
-    const double *f0 = C.lfac + J.mode0 * (C.T + 1), *f1 = C.lfac + J.mode1 * (C.T + 1);
-    Acc E = {0, 0, 0, 0}, O = {0, 0, 0, 0};
-    for (int par = 0; par < 2; ++par) {
-        const int nk = (C.T - m - par >= 0) ? (C.T - m - par) / 2 + 1 : 0;
-        for (int k0 = 0; k0 < nk; k0 += 4) {
-            const int k = k0 + kq;
-            const bool kv = k < nk;
-            const int l = m + par + 2 * k;
-            const int lm = lmb + (l - m);
-            const long trow = (long)lm * C.nhp + pt * 16 + i;
-            if (J.kind == SYN_SCALAR) {
-                const T a = kv ? tP[trow] : T(0);
-                const T b = (kv && cv) ? T(f0[l] * s0[2 * lm + part]) : T(0);
-                if (par == 0) E = P::mfma(a, b, E);
-                else O = P::mfma(a, b, O);
+    if (active) {
+        const int nl = C.T - m + 1;
+        const bool scalar = J.kind == SYN_SCALAR;
+        for (int seg = 0; seg < 8; ++seg) {
+            const double *src = nullptr, *fac = nullptr;
+            if (scalar) {
+                const int f = g * 8 + seg;
+                if (f < J.nitem) { src = J.sin0 + (long)f * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
             } else {
-                const T aD = kv ? tD[trow] : T(0);
-                const T aQ = kv ? tQ[trow] : T(0);
-                double sr = 0, si = 0, tr = 0, ti = 0;
-                if (kv && cv) {
-                    sr = f0[l] * s0[2 * lm];
-                    si = f0[l] * s0[2 * lm + 1];
-                    if (s1) {
-                        tr = f1[l] * s1[2 * lm];
-                        ti = f1[l] * s1[2 * lm + 1];
-                    }
+                const int it = g * 4 + (seg & 3);
+                if (it < J.nitem) {
+                    if (seg < 4) { src = J.sin0 + (long)it * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
+                    else if (J.kind == SYN_VECTOR) { src = J.sin1 + (long)it * J.sstride; fac = C.lfac + J.mode1 * (C.T + 1); }
                 }
+            }
+            for (int n = lane; n < nl; n += 64) {
+                cx<T> v{T(0), T(0)};
+                if (src) {
+                    const double f = fac[m + n];
+                    v = cx<T>{T(f * src[2 * (lmb + n)]), T(f * src[2 * (lmb + n) + 1])};
+                }
+                sb[seg * A.kmax + n] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int i = lane & 15, kq = lane >> 4;
+    const int part = i & 1;
+    const int comp = (i >> 1) & 1;   // vector jobs: 0 = V_theta column, 1 = V_phi column
+    const cx<T> *sf = sb + (J.kind == SYN_SCALAR ? (i >> 1) : (i >> 2)) * A.kmax;   // field / S segment
+    const cx<T> *st_ = sb + (4 + (i >> 2)) * A.kmax;                                 // T segment
+    const T *__restrict__ tP = (const T *)C.tabP;
+    const T *__restrict__ tD = (const T *)C.tabD;
+    const T *__restrict__ tQ = (const T *)C.tabQ;
+    const long tcol = pt * 16 + i;
+    Acc E = {0, 0, 0, 0}, O = {0, 0, 0, 0};
+    // MFMA steps of 4 degrees: the even-(l-m) steps first, then the odd ones.
+    const int nk0 = (C.T - m) / 2 + 1;
+    const int nk1 = (C.T - m - 1 >= 0) ? (C.T - m - 1) / 2 + 1 : 0;
+    const int s0n = (nk0 + 3) / 4, S = s0n + (nk1 + 3) / 4;
+    for (int c0 = 0; c0 < S; c0 += KCH_S) {
+        if (J.kind == SYN_SCALAR) {
+            T a[KCH_S], b[KCH_S];
+#pragma unroll
+            for (int u = 0; u < KCH_S; ++u) {
+                const int st = c0 + u;
+                const int par = st < s0n ? 0 : 1;
+                const int k = (st < s0n ? st : st - s0n) * 4 + kq;
+                const bool kv = st < S && k < (par ? nk1 : nk0);
+                const int n = par + 2 * k;
+                a[u] = kv ? tP[(long)(lmb + n) * C.nhp + tcol] : T(0);
+                const cx<T> c = kv ? sf[n] : cx<T>{T(0), T(0)};
+                b[u] = part ? c.y : c.x;
+            }
+#pragma unroll
+            for (int u = 0; u < KCH_S; ++u) {
+                const int st = c0 + u;
+                if (st < s0n) E = P::mfma(a[u], b[u], E);
+                else if (st < S) O = P::mfma(a[u], b[u], O);
+            }
+        } else {
+            T aD[KCH_S], aQ[KCH_S], bD[KCH_S], bQ[KCH_S];
+#pragma unroll
+            for (int u = 0; u < KCH_S; ++u) {
+                const int st = c0 + u;
+                const int par = st < s0n ? 0 : 1;
+                const int k = (st < s0n ? st : st - s0n) * 4 + kq;
+                const bool kv = st < S && k < (par ? nk1 : nk0);
+                const int n = par + 2 * k;
+                aD[u] = kv ? tD[(long)(lmb + n) * C.nhp + tcol] : T(0);
+                aQ[u] = kv ? tQ[(long)(lmb + n) * C.nhp + tcol] : T(0);
+                const cx<T> sv = kv ? sf[n] : cx<T>{T(0), T(0)};
+                const cx<T> tv = kv ? st_[n] : cx<T>{T(0), T(0)};
                 // V_theta = S*D + (iT)*Q ; V_phi = (iS)*Q + (-T)*D
-                const double bD = comp == 0 ? (part ? si : sr) : (part ? -ti : -tr);
-                const double bQ = comp == 0 ? (part ? tr : -ti) : (part ? sr : -si);
-                if (par == 0) {
-                    E = P::mfma(aQ, T(bQ), E);
-                    O = P::mfma(aD, T(bD), O);
-                } else {
-                    O = P::mfma(aQ, T(bQ), O);
-                    E = P::mfma(aD, T(bD), E);
+                bD[u] = comp == 0 ? (part ? sv.y : sv.x) : (part ? -tv.y : -tv.x);
+                bQ[u] = comp == 0 ? (part ? tv.x : -tv.y) : (part ? sv.x : -sv.y);
+            }
+#pragma unroll
+            for (int u = 0; u < KCH_S; ++u) {
+                const int st = c0 + u;
+                // P and Q have the parity of l-m, dP/dtheta the opposite one
+                if (st < s0n) {
+                    E = P::mfma(aQ[u], bQ[u], E);
+                    O = P::mfma(aD[u], bD[u], O);
+                } else if (st < S) {
+                    O = P::mfma(aQ[u], bQ[u], O);
+                    E = P::mfma(aD[u], bD[u], E);
                 }
             }
         }
@@ -202,10 +275,10 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     using P = Prec<PREC>;
     using T = typename P::T;
     using Acc = typename P::Acc;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int task = blockIdx.x * 4 + wave;
     if (task >= A.ntask) return;
-    const Job &J = A.job[find_job(A, task, true)];
+    const Job &J = job_of(A, task, true);
     const CfgDev &C = J.c;
     const int ntpp = ((C.T + 2) / 2 + 15) / 16;  // tiles per parity
     int t = task - J.ltask0;
@@ -220,9 +293,12 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     const int lmb = lm_base(C.T, m);
     const int kra = tile * 16 + i;
     const bool rv = kra < nk;
-    const long arow = (long)(lmb + par + 2 * kra) * C.nhp;
-    const T *tP = (const T *)C.tabP, *tD = (const T *)C.tabD, *tQ = (const T *)C.tabQ;
-    const T *four = (const T *)A.four + J.four_off;
+    // A[l][pair] from the transposed tables: 16 consecutive degrees per row of lanes
+    const long abase = (long)(m * 2 + par) * C.nhp * C.kp + kra;
+    const T *__restrict__ tP = (const T *)C.tabPT;
+    const T *__restrict__ tD = (const T *)C.tabDT;
+    const T *__restrict__ tQ = (const T *)C.tabQT;
+    const T *__restrict__ four = (const T *)A.four + J.four_off;
     const long mstride = (long)(C.T + 1) * C.nhp * 32;  // one intermediate group
     const long mo = (long)m * C.nhp * 32;
     Acc acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
@@ -235,11 +311,19 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     const int gpi = has_ek ? 3 : 2;  // grid fields per vector item
     if (J.kind == ANA_SCALAR) {
         const long cb = (long)(item >> 3) * mstride + mo + par * 16 + (item & 7) * 2 + part;
-        for (int p0 = 0; p0 < C.nhp; p0 += 4) {
-            const int pair = p0 + kq;
-            const T a = rv ? tP[arow + pair] : T(0);
-            const T b = cv ? four[cb + (long)pair * 32] : T(0);
-            acc = P::mfma(a, b, acc);
+        for (int p0 = 0; p0 < C.nhp; p0 += 4 * KCH) {
+            T a[KCH], b[KCH];
+#pragma unroll
+            for (int u = 0; u < KCH; ++u) {
+                const int pair = p0 + 4 * u + kq;
+                const bool pv = pair < C.nhp;
+                a[u] = (rv && pv) ? tP[abase + (long)pair * C.kp] : T(0);
+                b[u] = (cv && pv) ? four[cb + (long)pair * 32] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < KCH; ++u) {
+                if (p0 + 4 * u < C.nhp) acc = P::mfma(a[u], b[u], acc);
+            }
         }
     } else {
         const int gt = item * gpi, gp = gt + 1, ge = gt + 2;
@@ -247,28 +331,36 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
         const long bt = (long)(gt >> 3) * mstride + mo + (gt & 7) * 2;
         const long bp = (long)(gp >> 3) * mstride + mo + (gp & 7) * 2;
         const long be = (long)(ge >> 3) * mstride + mo + (ge & 7) * 2;
-        for (int p0 = 0; p0 < C.nhp; p0 += 4) {
-            const int pair = p0 + kq;
-            const long po = (long)pair * 32;
-            const T aD = rv ? tD[arow + pair] : T(0);
-            const T aQ = rv ? tQ[arow + pair] : T(0);
-            T bD = 0, bQ = 0, bE = 0;
-            if (cv) {
-                // S = D*Gt - i Q*Gp ; T = -i Q*Gt - D*Gp   (times 1/(l(l+1)) in the epilogue)
-                if (comp == 0) {
-                    bD = four[bt + po + eoD * 16 + part];
-                    bQ = part ? -four[bp + po + eoQ * 16] : four[bp + po + eoQ * 16 + 1];
-                    if (has_ek) bE = four[be + po + par * 16 + part];
-                } else {
-                    bD = -four[bp + po + eoD * 16 + part];
-                    bQ = part ? -four[bt + po + eoQ * 16] : four[bt + po + eoQ * 16 + 1];
+        for (int p0 = 0; p0 < C.nhp; p0 += 4 * KCH) {
+            T aD[KCH], aQ[KCH], aP[KCH], bD[KCH], bQ[KCH], bE[KCH];
+#pragma unroll
+            for (int u = 0; u < KCH; ++u) {
+                const int pair = p0 + 4 * u + kq;
+                const bool pv = pair < C.nhp;
+                const long po = (long)pair * 32;
+                aD[u] = (rv && pv) ? tD[abase + (long)pair * C.kp] : T(0);
+                aQ[u] = (rv && pv) ? tQ[abase + (long)pair * C.kp] : T(0);
+                aP[u] = (has_ek && rv && pv) ? tP[abase + (long)pair * C.kp] : T(0);
+                bD[u] = bQ[u] = bE[u] = T(0);
+                if (cv && pv) {
+                    // S = D*Gt - i Q*Gp ; T = -i Q*Gt - D*Gp   (times 1/(l(l+1)) in the epilogue)
+                    if (comp == 0) {
+                        bD[u] = four[bt + po + eoD * 16 + part];
+                        bQ[u] = part ? -four[bp + po + eoQ * 16] : four[bp + po + eoQ * 16 + 1];
+                        if (has_ek) bE[u] = four[be + po + par * 16 + part];
+                    } else {
+                        bD[u] = -four[bp + po + eoD * 16 + part];
+                        bQ[u] = part ? -four[bt + po + eoQ * 16] : four[bt + po + eoQ * 16 + 1];
+                    }
                 }
             }
-            acc = P::mfma(aD, bD, acc);
-            acc = P::mfma(aQ, bQ, acc);
-            if (has_ek) {
-                const T aP = rv ? tP[arow + pair] : T(0);
-                acc2 = P::mfma(aP, bE, acc2);
+#pragma unroll
+            for (int u = 0; u < KCH; ++u) {
+                if (p0 + 4 * u < C.nhp) {
+                    acc = P::mfma(aD[u], bD[u], acc);
+                    acc = P::mfma(aQ[u], bQ[u], acc);
+                    if (has_ek) acc2 = P::mfma(aP[u], bE[u], acc2);
+                }
             }
         }
     }
@@ -296,6 +388,11 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
 // and Ns = product of earlier radices reads x[j + r N/R], multiplies by
 // e^{sign 2 pi i r (j mod Ns)/(Ns R)}, does a radix-R DFT and writes to
 // (j div Ns) Ns R + (j mod Ns) + r Ns. Natural order in and out, ping-pong in LDS.
+//
+// The grid lengths of the standard truncations (64, 96, 128, 160, 192, 256, 320)
+// have compile-time radix plans (8 first, then 4, 2, 3, 5) and take their twiddles
+// from an LDS copy of e^{-2 pi i k/N}; any other 2/3/5-smooth length runs the
+// runtime-plan path.
 
 template <class T> __device__ inline cx<T> cmul(cx<T> a, cx<T> b) { return cx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 template <class T> __device__ inline cx<T> cadd(cx<T> a, cx<T> b) { return cx<T>{a.x + b.x, a.y + b.y}; }
@@ -316,6 +413,25 @@ template <int S, class T> __device__ inline void dft4(cx<T> *v)
     v[2] = csub(a, c);
     v[1] = cadd(b, sixz<S>(d));
     v[3] = csub(b, sixz<S>(d));
+}
+template <int S, class T> __device__ inline void dft8(cx<T> *v)
+{
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<S>(e);
+    dft4<S>(o);
+    const T h = T(0.70710678118654752440084436210484904);  // cos(pi/4)
+    // w^1 = h(1 + S i), w^2 = S i, w^3 = h(-1 + S i) with w = e^{S 2 pi i/8}
+    const cx<T> t1 = cx<T>{h * (o[1].x - S * o[1].y), h * (o[1].y + S * o[1].x)};
+    const cx<T> t2 = sixz<S>(o[2]);
+    const cx<T> t3 = cx<T>{h * (-o[3].x - S * o[3].y), h * (-o[3].y + S * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
 }
 template <int S, class T> __device__ inline void dft3(cx<T> *v)
 {
@@ -347,10 +463,97 @@ template <int S, class T> __device__ inline void dft5(cx<T> *v)
     v[2] = cadd(r2, i2);
     v[3] = csub(r2, i2);
 }
+template <int S, int R, class T> __device__ inline void dft(cx<T> *v)
+{
+    if constexpr (R == 2) dft2<S>(v);
+    else if constexpr (R == 3) dft3<S>(v);
+    else if constexpr (R == 4) dft4<S>(v);
+    else if constexpr (R == 5) dft5<S>(v);
+    else dft8<S>(v);
+}
 
-// rows complex sequences of length N, row r at a + r*N. Returns the buffer holding the result.
+struct RadixPlan {
+    int npass;
+    int r[AER_MAXPASS];
+};
+__host__ __device__ constexpr RadixPlan radix_plan(int n)
+{
+    RadixPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    while (n % 8 == 0 && p.npass < AER_MAXPASS) { p.r[p.npass++] = 8; n /= 8; }
+    while (n % 4 == 0 && p.npass < AER_MAXPASS) { p.r[p.npass++] = 4; n /= 4; }
+    while (n % 2 == 0 && p.npass < AER_MAXPASS) { p.r[p.npass++] = 2; n /= 2; }
+    while (n % 3 == 0 && p.npass < AER_MAXPASS) { p.r[p.npass++] = 3; n /= 3; }
+    while (n % 5 == 0 && p.npass < AER_MAXPASS) { p.r[p.npass++] = 5; n /= 5; }
+    return p;
+}
+__host__ __device__ constexpr int ns_before(int n, int pass)
+{
+    const RadixPlan p = radix_plan(n);
+    int s = 1;
+    for (int i = 0; i < pass; ++i) s *= p.r[i];
+    return s;
+}
+
+// Grid lengths with a compile-time FFT path, and the rows (complex sequences =
+// latitude-pair x field) one block of AER_FFT_THREADS transforms.
+__host__ __device__ constexpr bool fft_fixed(int n)
+{
+    return n == 64 || n == 96 || n == 128 || n == 160 || n == 192 || n == 256 || n == 320;
+}
+__host__ __device__ constexpr int fft_fixed_rows(int n) { return n == 64 ? 16 : 8; }
+
+template <int S, class T, int N, int ROWS, int P>
+__device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__restrict__ dst,
+                                      const cx<T> *__restrict__ tw)
+{
+    constexpr RadixPlan PL = radix_plan(N);
+    constexpr int R = PL.r[P];
+    constexpr int Ns = ns_before(N, P);
+    constexpr int NR = N / R;
+    constexpr int TS = N / (Ns * R);
+    constexpr int TOT = ROWS * NR;
+    constexpr int NT = AER_FFT_THREADS;
+#pragma unroll
+    for (int it = 0; it < (TOT + NT - 1) / NT; ++it) {
+        const int bi = threadIdx.x + it * NT;
+        if (TOT % NT != 0 && bi >= TOT) break;
+        const int rp = bi / NR, j = bi - rp * NR;
+        const int k = j % Ns;
+        cx<T> v[R];
+        const cx<T> *s = src + rp * N + j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = s[r * NR];
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cx<T> w = tw[r * k * TS];
+                if (S > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft<S, R>(v);
+        cx<T> *d = dst + rp * N + (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
+    }
+}
+
+template <int S, class T, int N, int ROWS, int P = 0>
+__device__ inline cx<T> *fft_block_fixed(cx<T> *a, cx<T> *b, const cx<T> *tw)
+{
+    constexpr RadixPlan PL = radix_plan(N);
+    if constexpr (P >= PL.npass) {
+        return a;
+    } else {
+        fft_pass_fixed<S, T, N, ROWS, P>(a, b, tw);
+        __syncthreads();
+        return fft_block_fixed<S, T, N, ROWS, P + 1>(b, a, tw);
+    }
+}
+
+// Runtime-plan path: rows complex sequences of length C.nlon. Returns the result buffer.
 template <int S, class T>
-__device__ cx<T> *fft_rows(cx<T> *a, cx<T> *b, int rows, const CfgDev &C)
+__device__ __forceinline__ cx<T> *fft_rows(cx<T> *a, cx<T> *b, int rows, const CfgDev &C)
 {
     using T2 = cx<T>;
     const int N = C.nlon;
@@ -364,7 +567,7 @@ __device__ cx<T> *fft_rows(cx<T> *a, cx<T> *b, int rows, const CfgDev &C)
             const T2 *src = a + rp * N;
             T2 *dst = b + rp * N;
             const int k = j % Ns;
-            T2 v[5];
+            T2 v[8];
             for (int r = 0; r < R; ++r) v[r] = src[j + r * NR];
             for (int r = 1; r < R; ++r) {
                 T2 w = twp[k * (R - 1) + r - 1];
@@ -375,7 +578,8 @@ __device__ cx<T> *fft_rows(cx<T> *a, cx<T> *b, int rows, const CfgDev &C)
                 case 2: dft2<S>(v); break;
                 case 3: dft3<S>(v); break;
                 case 4: dft4<S>(v); break;
-                default: dft5<S>(v); break;
+                case 5: dft5<S>(v); break;
+                default: dft8<S>(v); break;
             }
             const int base = (j / Ns) * Ns * R + k;
             for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
@@ -387,43 +591,146 @@ __device__ cx<T> *fft_rows(cx<T> *a, cx<T> *b, int rows, const CfgDev &C)
     return a;
 }
 
-// Synthesis FFT: one block per (job, group of 8 grid fields, fb of them, block of pb latitude pairs).
-template <int PREC>
-__global__ __launch_bounds__(256) void k_fft_synth(const LaunchArgs A)
+// Where a synthesis grid field goes: output pointer and scale.
+__device__ __forceinline__ double *syn_out(const Job &J, int gf, double &sc)
 {
-    using P = Prec<PREC>;
-    using T = typename P::T;
-    using T2 = typename P::T2;
-    extern __shared__ char smem_raw[];
-    const int task = blockIdx.x;
-    if (task >= A.ntask) return;
-    const Job &J = A.job[find_job(A, task, false)];
-    const CfgDev &C = J.c;
-    const int nsub = 8 / C.fb;
-    int t = task - J.ftask0;
-    const int g = t % J.ngroup;
-    t /= J.ngroup;
-    const int sub = t % nsub;
-    const int p0 = (t / nsub) * C.pb;
+    if (J.kind == SYN_SCALAR) { sc = J.gsc0; return J.gout0 + (long)gf * J.gstride; }
+    const int comp = gf & 1;
+    sc = comp ? J.gsc1 : J.gsc0;
+    return (comp ? J.gout1 : J.gout0) + (long)(gf >> 1) * J.gstride;
+}
+
+// Where an analysis grid field comes from.
+__device__ __forceinline__ const double *ana_in(const Job &J, int gf, double &sc)
+{
+    const int gpi = (J.kind == ANA_VECTOR) ? (J.gin2 ? 3 : 2) : 1;
+    const int comp = gf % gpi, item = gf / gpi;
+    sc = comp == 0 ? J.gsc0 : (comp == 1 ? J.gsc1 : 1.0);
+    return (comp == 0 ? J.gin0 : (comp == 1 ? J.gin1 : J.gin2)) + (long)item * J.gstride;
+}
+
+// Z_m = F_N + i F_S for 0 < m <= M, Z_{N-m} = conj(F_N) + i conj(F_S), Z_0 = Re F_N + i Re F_S,
+// with F_N = E + O, F_S = E - O read from the intermediate.
+template <class T>
+__device__ __forceinline__ void synth_pack(const T *__restrict__ b, int m, cx<T> &zm, cx<T> &zc)
+{
+    const T er = b[0], ei = b[1], or_ = b[16], oi = b[17];
+    const T nr = er + or_, ni = ei + oi, sr = er - or_, si = ei - oi;
+    if (m == 0) {
+        zm = cx<T>{nr, sr};
+        zc = zm;
+    } else {
+        zm = cx<T>{nr - si, ni + sr};
+        zc = cx<T>{nr + si, sr - ni};
+    }
+}
+
+// G_N = (Z_m + conj Z_{N-m})/2, G_S = -i (Z_m - conj Z_{N-m})/2; E = w (G_N + G_S), O = w (G_N - G_S).
+template <class T>
+__device__ inline void anal_unpack(cx<T> zm, cx<T> zc, T w, T *__restrict__ out)
+{
+    const T gnr = T(0.5) * (zm.x + zc.x), gni = T(0.5) * (zm.y - zc.y);
+    const T gsr = T(0.5) * (zm.y + zc.y), gsi = -T(0.5) * (zm.x - zc.x);
+    out[0] = w * (gnr + gsr);
+    out[1] = w * (gni + gsi);
+    out[16] = w * (gnr - gsr);
+    out[17] = w * (gni - gsi);
+}
+
+// ---- synthesis FFT, compile-time length: block = 8 grid fields x ROWS/8 latitude pairs,
+// row rp = pair_local * 8 + field_local.
+template <class T, int N, int ROWS>
+__device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, const T *__restrict__ four)
+{
+    constexpr int NT = AER_FFT_THREADS;
+    using T2 = cx<T>;
+    T2 *bufA = smem, *bufB = smem + ROWS * N, *tw = smem + 2 * ROWS * N;
+    const T2 *twg = (const T2 *)C.twN;
+    for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+    const int M = C.T;
+    const int nz = N - 2 * M - 1;
+    for (int idx = threadIdx.x; idx < ROWS * nz; idx += NT) {
+        const int rp = idx % ROWS, k = M + 1 + idx / ROWS;
+        bufA[rp * N + k] = T2{0, 0};
+    }
+    for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
+        const int rp = idx % ROWS, m = idx / ROWS;
+        const int gfl = rp & 7, pair = p0 + (rp >> 3);
+        T2 zm{0, 0}, zc{0, 0};
+        if (pair < C.nh && g * 8 + gfl < J.ngf)
+            synth_pack(four + ((long)m * C.nhp + pair) * 32 + gfl * 2, m, zm, zc);
+        bufA[rp * N + m] = zm;
+        if (m > 0) bufA[rp * N + N - m] = zc;
+    }
+    __syncthreads();
+    const T2 *res = fft_block_fixed<+1, T, N, ROWS>(bufA, bufB, tw);
+    for (int idx = threadIdx.x; idx < ROWS * N; idx += NT) {
+        const int rp = idx / N, i = idx - rp * N;
+        const int gfl = rp & 7, pair = p0 + (rp >> 3), gf = g * 8 + gfl;
+        if (pair >= C.nh || gf >= J.ngf) continue;
+        double sc;
+        double *out = syn_out(J, gf, sc);
+        const T2 z = res[idx];
+        const int js = C.nlat - 1 - pair;
+        out[(long)pair * N + i] = sc * (double)z.x;
+        if (js != pair) out[(long)js * N + i] = sc * (double)z.y;
+    }
+}
+
+template <class T, int N, int ROWS>
+__device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, T *__restrict__ four)
+{
+    constexpr int NT = AER_FFT_THREADS;
+    using T2 = cx<T>;
+    T2 *bufA = smem, *bufB = smem + ROWS * N, *tw = smem + 2 * ROWS * N;
+    const T2 *twg = (const T2 *)C.twN;
+    for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+    for (int idx = threadIdx.x; idx < ROWS * N; idx += NT) {
+        const int rp = idx / N, i = idx - rp * N;
+        const int gfl = rp & 7, pair = p0 + (rp >> 3), gf = g * 8 + gfl;
+        T2 z{0, 0};
+        if (pair < C.nh && gf < J.ngf) {
+            double sc;
+            const double *in = ana_in(J, gf, sc);
+            z = T2{T(sc * in[(long)pair * N + i]), T(sc * in[(long)(C.nlat - 1 - pair) * N + i])};
+        }
+        bufA[idx] = z;
+    }
+    __syncthreads();
+    const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
+    const int M = C.T;
+    for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
+        const int rp = idx % ROWS, m = idx / ROWS;
+        const int gfl = rp & 7, pair = p0 + (rp >> 3);
+        if (pair >= C.nhp) continue;
+        T *o = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
+        if (pair < C.nh && g * 8 + gfl < J.ngf) {
+            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(C.wpair[pair]), o);
+        } else {
+            o[0] = o[1] = o[16] = o[17] = T(0);
+        }
+    }
+}
+
+// ---- runtime-length paths (block = fb fields x pb pairs, row rp = field_local * pb + pair_local)
+template <class T>
+__device__ __forceinline__ void fft_synth_generic(const Job &J, const CfgDev &C, int g, int sub, int p0, cx<T> *smem, const T *__restrict__ four)
+{
+    using T2 = cx<T>;
     const int N = C.nlon, M = C.T, rows = C.fb * C.pb;
-    T2 *bufA = (T2 *)smem_raw, *bufB = bufA + rows * N;
-    const T *four = (const T *)A.four + J.four_off + (long)g * (M + 1) * C.nhp * 32;
+    T2 *bufA = smem, *bufB = smem + rows * N;
     for (int idx = threadIdx.x; idx < rows * N; idx += blockDim.x) {
         const int rp = idx / N, k = idx - rp * N;
         const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb;
-        T2 z = T2{0, 0};
+        T2 z{0, 0};
         if (pair < C.nh && g * 8 + gfl < J.ngf) {
             int m = -1;
-            bool cj = false;
             if (k <= M) m = k;
-            else if (k >= N - M) { m = N - k; cj = true; }
+            else if (k >= N - M) m = N - k;
             if (m >= 0) {
-                const long b = ((long)m * C.nhp + pair) * 32 + gfl * 2;
-                const T er = four[b], ei = four[b + 1], or_ = four[b + 16], oi = four[b + 17];
-                const T nr = er + or_, ni = ei + oi, sr = er - or_, si = ei - oi;
-                if (k == 0) z = T2{nr, sr};
-                else if (!cj) z = T2{nr - si, ni + sr};   // F_N + i F_S
-                else z = T2{nr + si, sr - ni};            // conj(F_N) + i conj(F_S)
+                T2 zm, zc;
+                synth_pack(four + ((long)m * C.nhp + pair) * 32 + gfl * 2, m, zm, zc);
+                z = (k <= M) ? zm : zc;
             }
         }
         bufA[idx] = z;
@@ -432,17 +739,10 @@ __global__ __launch_bounds__(256) void k_fft_synth(const LaunchArgs A)
     const T2 *res = fft_rows<+1, T>(bufA, bufB, rows, C);
     for (int idx = threadIdx.x; idx < rows * N; idx += blockDim.x) {
         const int rp = idx / N, i = idx - rp * N;
-        const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb;
-        const int gf = g * 8 + gfl;
+        const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb, gf = g * 8 + gfl;
         if (pair >= C.nh || gf >= J.ngf) continue;
-        double *out;
         double sc;
-        if (J.kind == SYN_SCALAR) { out = J.gout0 + (long)gf * J.gstride; sc = J.gsc0; }
-        else {
-            const int comp = gf & 1;
-            out = (comp ? J.gout1 : J.gout0) + (long)(gf >> 1) * J.gstride;
-            sc = comp ? J.gsc1 : J.gsc0;
-        }
+        double *out = syn_out(J, gf, sc);
         const T2 z = res[idx];
         const int js = C.nlat - 1 - pair;
         out[(long)pair * N + i] = sc * (double)z.x;
@@ -450,66 +750,99 @@ __global__ __launch_bounds__(256) void k_fft_synth(const LaunchArgs A)
     }
 }
 
-// Analysis FFT: rows north + i south, forward transform, unpack, fold to E/O, weight.
-template <int PREC>
-__global__ __launch_bounds__(256) void k_fft_anal(const LaunchArgs A)
+template <class T>
+__device__ __forceinline__ void fft_anal_generic(const Job &J, const CfgDev &C, int g, int sub, int p0, cx<T> *smem, T *__restrict__ four)
 {
-    using P = Prec<PREC>;
-    using T = typename P::T;
-    using T2 = typename P::T2;
-    extern __shared__ char smem_raw[];
-    const int task = blockIdx.x;
-    if (task >= A.ntask) return;
-    const Job &J = A.job[find_job(A, task, false)];
-    const CfgDev &C = J.c;
-    const int nsub = 8 / C.fb;
-    int t = task - J.ftask0;
-    const int g = t % J.ngroup;
-    t /= J.ngroup;
-    const int sub = t % nsub;
-    const int p0 = (t / nsub) * C.pb;
+    using T2 = cx<T>;
     const int N = C.nlon, M = C.T, rows = C.fb * C.pb;
-    T2 *bufA = (T2 *)smem_raw, *bufB = bufA + rows * N;
-    const int gpi = (J.kind == ANA_VECTOR) ? (J.gin2 ? 3 : 2) : 1;
+    T2 *bufA = smem, *bufB = smem + rows * N;
     for (int idx = threadIdx.x; idx < rows * N; idx += blockDim.x) {
         const int rp = idx / N, i = idx - rp * N;
-        const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb;
-        const int gf = g * 8 + gfl;
-        T2 z = T2{0, 0};
+        const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb, gf = g * 8 + gfl;
+        T2 z{0, 0};
         if (pair < C.nh && gf < J.ngf) {
-            const int comp = gf % gpi, item = gf / gpi;
-            const double *in = comp == 0 ? J.gin0 : (comp == 1 ? J.gin1 : J.gin2);
-            const double sc = comp == 0 ? J.gsc0 : (comp == 1 ? J.gsc1 : 1.0);
-            in += (long)item * J.gstride;
+            double sc;
+            const double *in = ana_in(J, gf, sc);
             z = T2{T(sc * in[(long)pair * N + i]), T(sc * in[(long)(C.nlat - 1 - pair) * N + i])};
         }
         bufA[idx] = z;
     }
     __syncthreads();
     const T2 *res = fft_rows<-1, T>(bufA, bufB, rows, C);
-    T *four = (T *)A.four + J.four_off + (long)g * (M + 1) * C.nhp * 32;
     for (int idx = threadIdx.x; idx < rows * (M + 1); idx += blockDim.x) {
         const int rp = idx / (M + 1), m = idx - rp * (M + 1);
         const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb;
         if (pair >= C.nhp) continue;
-        T er = 0, ei = 0, or_ = 0, oi = 0;
-        if (pair < C.nh && g * 8 + gfl < J.ngf) {
-            const T2 zm = res[rp * N + m];
-            const T2 zc = res[rp * N + (m == 0 ? 0 : N - m)];
-            // G_N = (Z_m + conj Z_{N-m})/2 ; G_S = -i (Z_m - conj Z_{N-m})/2
-            const T gnr = T(0.5) * (zm.x + zc.x), gni = T(0.5) * (zm.y - zc.y);
-            const T gsr = T(0.5) * (zm.y + zc.y), gsi = -T(0.5) * (zm.x - zc.x);
-            const T w = T(C.wpair[pair]);
-            er = w * (gnr + gsr);
-            ei = w * (gni + gsi);
-            or_ = w * (gnr - gsr);
-            oi = w * (gni - gsi);
-        }
-        const long b = ((long)m * C.nhp + pair) * 32 + gfl * 2;
-        four[b] = er;
-        four[b + 1] = ei;
-        four[b + 16] = or_;
-        four[b + 17] = oi;
+        T *o = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
+        if (pair < C.nh && g * 8 + gfl < J.ngf)
+            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(C.wpair[pair]), o);
+        else
+            o[0] = o[1] = o[16] = o[17] = T(0);
+    }
+}
+
+// Block -> (job, intermediate group, field subgroup, first latitude pair).
+__device__ __forceinline__ const Job &fft_task(const LaunchArgs &A, int task, int &g, int &sub, int &p0)
+{
+    const Job &J = job_of(A, task, false);
+    const int nsub = 8 / J.c.fb;
+    int t = task - J.ftask0;
+    g = t % J.ngroup;
+    t /= J.ngroup;
+    sub = t % nsub;
+    p0 = (t / nsub) * J.c.pb;
+    return J;
+}
+
+#define AER_FFT_DISPATCH(BODY, ...)                                                        \
+    switch (C.nlon) {                                                                      \
+        case 64: BODY##_fixed<T, 64, fft_fixed_rows(64)>(__VA_ARGS__); break;              \
+        case 96: BODY##_fixed<T, 96, fft_fixed_rows(96)>(__VA_ARGS__); break;              \
+        case 128: BODY##_fixed<T, 128, fft_fixed_rows(128)>(__VA_ARGS__); break;           \
+        case 160: BODY##_fixed<T, 160, fft_fixed_rows(160)>(__VA_ARGS__); break;           \
+        case 192: BODY##_fixed<T, 192, fft_fixed_rows(192)>(__VA_ARGS__); break;           \
+        case 256: BODY##_fixed<T, 256, fft_fixed_rows(256)>(__VA_ARGS__); break;           \
+        case 320: BODY##_fixed<T, 320, fft_fixed_rows(320)>(__VA_ARGS__); break;           \
+        default: break;                                                                    \
+    }
+
+// GENERIC = false: only the compile-time lengths (no scratch); true: also the
+// runtime-plan path (the host picks it when some job has another grid length).
+template <int PREC, bool GENERIC>
+__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_synth(const LaunchArgs A)
+{
+    using T = typename Prec<PREC>::T;
+    extern __shared__ char smem_raw[];
+    const int task = blockIdx.x;
+    if (task >= A.ntask) return;
+    int g, sub, p0;
+    const Job &J = fft_task(A, task, g, sub, p0);
+    const CfgDev &C = J.c;
+    cx<T> *smem = (cx<T> *)smem_raw;
+    const T *four = (const T *)A.four + J.four_off + (long)g * (C.T + 1) * C.nhp * 32;
+    if (fft_fixed(C.nlon)) {
+        AER_FFT_DISPATCH(fft_synth, J, C, g, p0, smem, four)
+    } else if constexpr (GENERIC) {
+        fft_synth_generic<T>(J, C, g, sub, p0, smem, four);
+    }
+}
+
+template <int PREC, bool GENERIC>
+__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_anal(const LaunchArgs A)
+{
+    using T = typename Prec<PREC>::T;
+    extern __shared__ char smem_raw[];
+    const int task = blockIdx.x;
+    if (task >= A.ntask) return;
+    int g, sub, p0;
+    const Job &J = fft_task(A, task, g, sub, p0);
+    const CfgDev &C = J.c;
+    cx<T> *smem = (cx<T> *)smem_raw;
+    T *four = (T *)A.four + J.four_off + (long)g * (C.T + 1) * C.nhp * 32;
+    if (fft_fixed(C.nlon)) {
+        AER_FFT_DISPATCH(fft_anal, J, C, g, p0, smem, four)
+    } else if constexpr (GENERIC) {
+        fft_anal_generic<T>(J, C, g, sub, p0, smem, four);
     }
 }
 

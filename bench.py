@@ -97,7 +97,41 @@ class TransformSet:
         self.sout["lnps"] = torch.empty((nlm,), dtype=torch.complex128, device="cuda")
         self.q_t = s()
 
+    def _jobs(self):
+        """The set as two fused job lists: 8 synthesis jobs, 4 analysis jobs."""
+        from aeros_amd import _lib as L
+        o, so, a = self.gout, self.sout, 1.0 / 6.371e6
+        P = lambda t: t.data_ptr()
+        J = lambda kind, n, ins, outs, m=(0, 0, 0), sc=(0.0, 0.0): L.AerJob(
+            kind, n, *(list(map(P, ins)) + [0] * (3 - len(ins))), *(list(map(P, outs)) + [0] * (2 - len(outs))),
+            0, 0, m[0], m[1], m[2], sc[0], sc[1], 0)
+        syn = [J(L.AER_JOB_SYNTH_SCALAR, 1, [self.lnps], [o["lnps_g"]]),
+               J(L.AER_JOB_SYNTH_GRAD, 1, [self.lnps], [o["dlnpsdy"], o["dlnpsdx"]], sc=(-a, a)),
+               J(L.AER_JOB_SYNTH_VECTOR, NLEV, [self.div, self.vor], [o["v"], o["u"]],
+                 m=(L.AER_MODE_S_FROM_DIV, L.AER_MODE_T_FROM_VOR, 0), sc=(-1.0, 1.0)),
+               J(L.AER_JOB_SYNTH_SCALAR, NLEV, [self.vor], [o["vor_g"]]),
+               J(L.AER_JOB_SYNTH_SCALAR, NLEV, [self.div], [o["div_g"]]),
+               J(L.AER_JOB_SYNTH_SCALAR, NLEV, [self.temp], [o["t_g"]]),
+               J(L.AER_JOB_SYNTH_GRAD, NLEV, [self.temp], [o["dtdy"], o["dtdx"]], sc=(-a, a)),
+               J(L.AER_JOB_SYNTH_SCALAR, NLEV, [self.q], [self.q_g])]
+        ana = [J(L.AER_JOB_ANAL_VECTOR, NLEV, [self.an, self.ae, self.ek], [so["div"], so["vor"]],
+                 m=(L.AER_MODE_DIV_FROM_S, L.AER_MODE_VOR_FROM_T, L.AER_MODE_NEG_LAP), sc=(-1.0, 1.0)),
+               J(L.AER_JOB_ANAL_SCALAR, NLEV, [self.dtdt], [so["temp"]]),
+               J(L.AER_JOB_ANAL_SCALAR, 1, [self.dlnpsdt], [so["lnps"]]),
+               J(L.AER_JOB_ANAL_SCALAR, NLEV, [self.qdot], [self.q_t])]
+        return (L.AerJob * len(syn))(*syn), len(syn), (L.AerJob * len(ana))(*ana), len(ana)
+
     def run_batched(self):
+        """One fused launch pair per direction (the whole set = 4 kernels)."""
+        from aeros_amd._lib import check, lib
+        if not hasattr(self, "_jl"):
+            self._jl = self._jobs()
+        js, ns, ja, na = self._jl
+        check(lib().aer_run_jobs_d(self.sht._h, ns, js))
+        check(lib().aer_run_jobs_d(self.sht._h, na, ja))
+
+    def run_calls(self):
+        """The same set through the per-purpose entry points (4 calls, 8 kernels)."""
         from aeros_amd._lib import check, lib
         sh, L = self.sht, NLEV
         h, ss, gs = sh._h, 2 * sh.nlm, sh.nlon * sh.nlat
@@ -160,6 +194,19 @@ def time_steps(torch, fn, steps, warmup):
     return e0.elapsed_time(e1) / steps   # ms per step
 
 
+def capture_graph(torch, sht, fn):
+    """Record one set into a HIP graph (torch.cuda.CUDAGraph over the library's launches)."""
+    fn()                      # first call of this shape allocates the intermediate
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        sht._bind_stream()    # the capture stream
+        fn()
+    torch.cuda.synchronize()
+    sht._bind_stream()        # back to the ambient stream
+    return g
+
+
 def kernel_times(sht, fn, steps):
     """Per-kernel-kind milliseconds per set and launches per set, from HIP events."""
     from aeros_amd._lib import check, lib
@@ -179,6 +226,84 @@ def rel_err(a, ref):
     if a.ndim >= 2 and a.shape[0] == NLEV:
         return max(float(np.abs(a[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in range(NLEV))
     return float(np.abs(a - ref).max() / np.abs(ref).max())
+
+
+def sref_bytes(nlon, nlat, nlm, L):
+    """Algorithmic bytes of one S_ref set (the reference's dry step, 11L+4 transforms)."""
+    G, S = 8 * nlon * nlat, 16 * nlm
+    Ns, Nv, Ng = 5 * L + 2, 2 * L, L + 1
+    return Ns * (G + S) + 2 * Nv * (G + S) + Ng * (S + 2 * G)
+
+
+def ensemble_config2(torch, precision, steps=50, warmup=5):
+    """64 Held-Suarez members of uneven size (24 x T21L16, 24 x T31L20, 16 x T42L20): the
+    S_ref transform set of every member in one compiled plan (4 launches), replayed from a
+    HIP graph. Returns throughput and the HBM rate of the algorithmic traffic."""
+    from aeros_amd import _lib as L
+    from aeros_amd.seeding import red_spectra, red_spectrum
+    from aeros_amd.spectral import AerosSht
+    mix = [(21, 16, 24), (31, 20, 24), (42, 20, 16)]
+    shts = {T: AerosSht(T, precision=precision) for T, _, _ in mix}
+    cfgs, jobs, keep, nbytes = [], [], [], 0
+    a = 1.0 / 6.371e6
+    P = lambda t: t.data_ptr()
+    J = lambda kind, n, ins, outs, md=(0, 0, 0), sc=(0.0, 0.0): L.AerJob(
+        kind, n, *(list(map(P, ins)) + [0] * (3 - len(ins))), *(list(map(P, outs)) + [0] * (2 - len(outs))),
+        0, 0, *md, *sc, 0)
+    member = 0
+    for T, nlev, count in mix:
+        s = shts[T]
+        for _ in range(count):
+            dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+            vor = dev(red_spectra(T, nlev, 2, member=member, field0=0, zero_l0=True) * 1e-5)
+            div = dev(red_spectra(T, nlev, 2, member=member, field0=100, zero_l0=True) * 1e-6)
+            tem = dev(red_spectra(T, nlev, 2, member=member, field0=200))
+            lnp = dev(red_spectrum(T, 2000 + 300 + 100000 * member) * 1e-2)
+            G = lambda n=nlev: torch.empty((n, s.nlat, s.nlon), dtype=torch.float64, device="cuda")
+            S = lambda n=nlev: torch.empty((n, s.nlm), dtype=torch.complex128, device="cuda")
+            u, v, zg, dg, tg, tx, ty = G(), G(), G(), G(), G(), G(), G()
+            lg, lx, ly = G(1), G(1), G(1)
+            ae, an, ek, dt = (torch.randn((nlev, s.nlat, s.nlon), dtype=torch.float64, device="cuda") for _ in range(4))
+            dl = torch.randn((1, s.nlat, s.nlon), dtype=torch.float64, device="cuda")
+            tv, td, tt, tl = S(), S(), S(), S(1)
+            keep += [vor, div, tem, lnp, u, v, zg, dg, tg, tx, ty, lg, lx, ly, ae, an, ek, dt, dl, tv, td, tt, tl]
+            mj = [J(L.AER_JOB_SYNTH_SCALAR, 1, [lnp], [lg]),
+                  J(L.AER_JOB_SYNTH_GRAD, 1, [lnp], [ly, lx], sc=(-a, a)),
+                  J(L.AER_JOB_SYNTH_VECTOR, nlev, [div, vor], [v, u],
+                    (L.AER_MODE_S_FROM_DIV, L.AER_MODE_T_FROM_VOR, 0), (-1.0, 1.0)),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [vor], [zg]),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [div], [dg]),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [tem], [tg]),
+                  J(L.AER_JOB_SYNTH_GRAD, nlev, [tem], [ty, tx], sc=(-a, a)),
+                  J(L.AER_JOB_ANAL_VECTOR, nlev, [an, ae, ek], [td, tv],
+                    (L.AER_MODE_DIV_FROM_S, L.AER_MODE_VOR_FROM_T, L.AER_MODE_NEG_LAP), (-1.0, 1.0)),
+                  J(L.AER_JOB_ANAL_SCALAR, nlev, [dt], [tt]),
+                  J(L.AER_JOB_ANAL_SCALAR, 1, [dl], [tl])]
+            cfgs += [s._h] * len(mj)
+            jobs += mj
+            nbytes += sref_bytes(s.nlon, s.nlat, s.nlm, nlev)
+            member += 1
+    n = len(jobs)
+    plan = ctypes.c_void_p()
+    L.check(L.lib().aer_plan_create(n, (ctypes.c_void_p * n)(*cfgs), (L.AerJob * n)(*jobs), ctypes.byref(plan)))
+    run = lambda: L.check(L.lib().aer_plan_run(plan, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    run()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run()
+    torch.cuda.synchronize()
+    ms = time_steps(torch, g.replay, steps, warmup)
+    L.lib().aer_plan_destroy(plan)
+    ntr = sum(count * (11 * nlev + 4) for _, nlev, count in mix)
+    return {"workload": "64 members: 24 x T21L16 + 24 x T31L20 + 16 x T42L20, S_ref set (11L+4 transforms) each, "
+                        "one compiled plan (4 launches) replayed from a HIP graph",
+            "precision": precision, "jobs": n, "ms_per_step": round(ms, 4),
+            "ensemble_steps_per_s": round(1000.0 / ms, 1),
+            "scalar_eq_transforms_per_s": round(ntr * 1000.0 / ms, 0),
+            "alg_bytes_per_step": int(nbytes),
+            "hbm_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
 def cpu_baseline(budget_s=12.0):
@@ -207,6 +332,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--only", choices=["set", "ensemble"], default=None,
+                    help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms) or only the ensemble")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -226,8 +353,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 8, "launches": 5}))
         return
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup)}))
+        return
     sht = AerosSht(T)
     ts = TransformSet(sht, torch)
+    if args.only == "set":
+        sht.set_precision(args.precision)
+        ms = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+        print(json.dumps({"set_ms_per_step": ms, "precision": args.precision}))
+        return
     sht._bind_stream()
 
     # FP64 reference outputs and the FP64 forms
@@ -238,13 +373,21 @@ def main():
     # the per-level tendency call also re-analyses dlnps/dt on every level: 20 extra
     # single-level transforms beyond the set's 264; normalize the plain time to 264
     ms_plain = time_steps(torch, ts.run_plain, plain_steps, 2) * (13 * NLEV + 4) / (14 * NLEV + 3)
-    ms_fp64 = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    ms_fp64_eager = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    g64 = capture_graph(torch, sht, ts.run_batched)
+    ms_fp64 = time_steps(torch, g64.replay, args.steps, args.warmup)
     k64_ms, _ = kernel_times(sht, ts.run_batched, max(10, args.steps // 4))
 
     sht.set_precision(args.precision)
     ts.run_batched()
     errs = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
-    ms = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    ms_eager = time_steps(torch, ts.run_batched, args.steps, args.warmup)
+    ms_calls = time_steps(torch, ts.run_calls, args.steps, args.warmup)
+    graph = capture_graph(torch, sht, ts.run_batched)
+    ms = time_steps(torch, graph.replay, args.steps, args.warmup)
+    torch.cuda.synchronize()
+    graph_errs = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
+    assert max(graph_errs.values()) <= max(errs.values()) * 1.0001 + 1e-300, "graph replay differs from eager"
     kms, klaunch = kernel_times(sht, ts.run_batched, max(10, args.steps // 4))
 
     bytes_k, flops_k = algorithmic(sht.nlon, sht.nlat, sht.nlm, NLEV)
@@ -284,7 +427,11 @@ def main():
                    "trunc": T, "nlev": NLEV, "grid": f"{sht.nlon}x{sht.nlat}",
                    "precision": "fp32-mfma" if args.precision == "fp32" else "fp64-mfma",
                    "io": "fp64 in HBM"},
+        "timing": "HIP graph replay of one set per step (4 kernels: fused synthesis and analysis launch pairs)",
+        "eager_sets_per_s": round(1000.0 / ms_eager, 2),
+        "eager_4_calls_sets_per_s": round(1000.0 / ms_calls, 2),
         "fp64_batched_sets_per_s": round(1000.0 / ms_fp64, 2),
+        "fp64_batched_eager_sets_per_s": round(1000.0 / ms_fp64_eager, 2),
         "fp64_plain_sets_per_s": round(1000.0 / ms_plain, 2),
         "speedup_vs_fp64_plain": round(ms_plain / ms, 2),
         "speedup_vs_fp64_batched": round(ms_fp64 / ms, 3),
@@ -294,6 +441,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
+    line["ensemble_config2"] = ensemble_config2(torch, args.precision)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

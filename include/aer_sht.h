@@ -29,6 +29,8 @@
 #ifndef AER_SHT_H
 #define AER_SHT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -159,6 +161,69 @@ int aer_tendency_spectral_d(aer_sht_t cfg, int nlev,
         const double *ae, const double *an, const double *ek, const double *dtdt,
         const double *dlnpsdt,
         double *tvor, double *tdiv, double *ttemp, double *tlnps);
+
+/* ---- fused launches: a list of batched transforms of one direction run as ONE
+ * launch per stage (the Legendre GEMM kernel and the FFT kernel). Every entry
+ * point above is such a list; this is the general form, used to put all the
+ * transforms of a model step (for example the tendency halves together with
+ * passive-scalar transforms) into one launch pair, and to record that launch
+ * pair into a HIP graph (no allocation happens inside after the first call of a
+ * given shape). No reference counterpart: the reference loops over levels and
+ * fields with one SHTns call each (src/dynamics/aeros_tendency.f90:409-419, 449-465). */
+#define AER_JOB_SYNTH_SCALAR 0   /* in0 spectral -> out0 grid */
+#define AER_JOB_SYNTH_VECTOR 1   /* (in0 = S, in1 = T) -> (out0 = V_theta, out1 = V_phi) */
+#define AER_JOB_SYNTH_GRAD   2   /* in0 -> (out0 = d/dtheta, out1 = (1/sin) d/dphi) */
+#define AER_JOB_ANAL_SCALAR  3   /* in0 grid -> out0 spectral */
+#define AER_JOB_ANAL_VECTOR  4   /* (in0 = V_theta, in1 = V_phi[, in2 = X]) -> (out0 = S, out1 = T) */
+
+/* Per-degree factors on spectral inputs (synthesis) or outputs (analysis). */
+#define AER_MODE_ONE        0   /* 1 */
+#define AER_MODE_S_FROM_DIV 1   /* -a/(l(l+1)), 0 at l = 0 */
+#define AER_MODE_T_FROM_VOR 2   /* +a/(l(l+1)), 0 at l = 0 */
+#define AER_MODE_INV_LL1    3   /* 1/(l(l+1)), 0 at l = 0 */
+#define AER_MODE_DIV_FROM_S 4   /* -1/a after the 1/(l(l+1)) of the vector analysis, 0 at l = 0 */
+#define AER_MODE_VOR_FROM_T 5   /* +1/a likewise */
+#define AER_MODE_NEG_LAP    6   /* +l(l+1)/a^2 (minus the Laplacian) */
+#define AER_MODE_ZERO       7   /* 0 */
+
+typedef struct aer_job {
+    int kind;              /* AER_JOB_* */
+    int nfield;            /* scalar fields, or vector pairs */
+    const double *in0, *in1, *in2;
+    double *out0, *out1;
+    long in_stride, out_stride;   /* doubles between consecutive fields; 0 = dense */
+    int mode0, mode1, mode2;      /* AER_MODE_*: spectral factor of in0/in1 (synthesis),
+                                     of out0/out1 (analysis); mode2 applies to the scalar
+                                     analysis of in2, which is added into out0 */
+    double scale0, scale1;        /* grid-side factor of component 0 / 1 (0 means 1) */
+    int accumulate;               /* analysis: add into out0/out1 instead of overwriting */
+} aer_job;
+
+/* All jobs must be of one direction (all AER_JOB_SYNTH_* or all AER_JOB_ANAL_*);
+ * at most AER_MAX_JOBS jobs per call. Asynchronous on the handle's stream. */
+#define AER_MAX_JOBS 8
+int aer_run_jobs_d(aer_sht_t cfg, int njob, const aer_job *jobs);
+
+/* ---- compiled plans: a fixed list of jobs, each on its own handle (so members of
+ * an ensemble may differ in truncation and level count), built once into
+ * device-resident job tables. Running a plan is at most four launches (synthesis
+ * Legendre + FFT, then analysis FFT + Legendre) on the given stream, with no host
+ * work beyond the launches and no allocation, so it can be captured in a HIP graph.
+ * All handles must have the same precision (taken at creation); arrays named in
+ * the jobs must stay valid while the plan is used. No reference counterpart: the
+ * reference has no ensemble or batching layer (its parallel unit is one transform
+ * per OpenMP thread, src/aeros_spectral.f90:87-113). */
+typedef struct aer_plan *aer_plan_t;
+int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_plan_t *plan);
+int aer_plan_run(aer_plan_t plan, void *stream);
+int aer_plan_destroy(aer_plan_t plan);
+
+/* ---- device memory for hosts without a HIP binding of their own (Fortran): thin
+ * wrappers of hipMalloc / hipFree / synchronous hipMemcpy. No reference counterpart. */
+int aer_dev_malloc(size_t bytes, void **ptr);
+int aer_dev_free(void *ptr);
+int aer_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int aer_memcpy_d2h(void *dst, const void *src, size_t bytes);
 
 /* Block until all work queued on the handle's stream has finished. */
 int aer_sht_sync(aer_sht_t cfg);

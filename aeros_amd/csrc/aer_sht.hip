@@ -362,8 +362,9 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
             B.ftask += J.ngroup * nsub * ((C.nhp + C.pb - 1) / C.pb);
         }
         const size_t cxb = 2 * sizeof(T);
+        const size_t rn = (size_t)aer::fft_fixed_rows(h->nlon) * h->nlon;
         const size_t lds = aer::fft_fixed(h->nlon)
-                               ? ((size_t)2 * aer::fft_fixed_rows(h->nlon) + 1) * h->nlon * cxb
+                               ? (2 * (rn + rn / 16 + 1) + h->nlon) * cxb
                                : (size_t)2 * h->fb * h->pb * h->nlon * cxb;
         B.lds = std::max(B.lds, lds);
         B.generic = B.generic || !aer::fft_fixed(h->nlon);

@@ -193,12 +193,22 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
     const int i = lane & 15, kq = lane >> 4;
     const int part = i & 1;
     const int comp = (i >> 1) & 1;   // vector jobs: 0 = V_theta column, 1 = V_phi column
-    const cx<T> *sf = sb + (J.kind == SYN_SCALAR ? (i >> 1) : (i >> 2)) * A.kmax;   // field / S segment
-    const cx<T> *st_ = sb + (4 + (i >> 2)) * A.kmax;                                 // T segment
-    const T *__restrict__ tP = (const T *)C.tabP;
-    const T *__restrict__ tD = (const T *)C.tabD;
-    const T *__restrict__ tQ = (const T *)C.tabQ;
-    const long tcol = pt * 16 + i;
+    // B operands as one LDS element each: b = sign * seg[2 (l-m) + offset]. Scalar: the
+    // field's real or imaginary part. Vector, with V_theta = S*D + (iT)*Q and
+    // V_phi = (iS)*Q + (-T)*D: bD = +-S or -T component, bQ = the other component of
+    // iT or iS, whose real part is -Im and imaginary part +Re.
+    const T *segS = (const T *)(sb + (J.kind == SYN_SCALAR ? (i >> 1) : (i >> 2)) * A.kmax);
+    const T *segT = (const T *)(sb + (4 + (i >> 2)) * A.kmax);
+    // Scalar jobs have no component: every lane reads its own field's segment.
+    const bool useS = J.kind == SYN_SCALAR || comp == 0;
+    const T *pD = (useS ? segS : segT) + part;
+    const T sD = comp == 0 ? T(1) : T(-1);
+    const T *pQ = (comp == 0 ? segT : segS) + (1 - part);
+    const T sQ = part ? T(1) : T(-1);
+    const int nhp = C.nhp;
+    const T *__restrict__ tP = (const T *)C.tabP + (long)lmb * nhp + pt * 16 + i;
+    const T *__restrict__ tD = (const T *)C.tabD + (long)lmb * nhp + pt * 16 + i;
+    const T *__restrict__ tQ = (const T *)C.tabQ + (long)lmb * nhp + pt * 16 + i;
     Acc E = {0, 0, 0, 0}, O = {0, 0, 0, 0};
     // MFMA steps of 4 degrees: the even-(l-m) steps first, then the odd ones.
     const int nk0 = (C.T - m) / 2 + 1;
@@ -214,9 +224,8 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
                 const int k = (st < s0n ? st : st - s0n) * 4 + kq;
                 const bool kv = st < S && k < (par ? nk1 : nk0);
                 const int n = par + 2 * k;
-                a[u] = kv ? tP[(long)(lmb + n) * C.nhp + tcol] : T(0);
-                const cx<T> c = kv ? sf[n] : cx<T>{T(0), T(0)};
-                b[u] = part ? c.y : c.x;
+                a[u] = kv ? tP[n * nhp] : T(0);
+                b[u] = kv ? pD[2 * n] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < KCH_S; ++u) {
@@ -233,13 +242,10 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
                 const int k = (st < s0n ? st : st - s0n) * 4 + kq;
                 const bool kv = st < S && k < (par ? nk1 : nk0);
                 const int n = par + 2 * k;
-                aD[u] = kv ? tD[(long)(lmb + n) * C.nhp + tcol] : T(0);
-                aQ[u] = kv ? tQ[(long)(lmb + n) * C.nhp + tcol] : T(0);
-                const cx<T> sv = kv ? sf[n] : cx<T>{T(0), T(0)};
-                const cx<T> tv = kv ? st_[n] : cx<T>{T(0), T(0)};
-                // V_theta = S*D + (iT)*Q ; V_phi = (iS)*Q + (-T)*D
-                bD[u] = comp == 0 ? (part ? sv.y : sv.x) : (part ? -tv.y : -tv.x);
-                bQ[u] = comp == 0 ? (part ? tv.x : -tv.y) : (part ? sv.x : -sv.y);
+                aD[u] = kv ? tD[n * nhp] : T(0);
+                aQ[u] = kv ? tQ[n * nhp] : T(0);
+                bD[u] = kv ? sD * pD[2 * n] : T(0);
+                bQ[u] = kv ? sQ * pQ[2 * n] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < KCH_S; ++u) {
@@ -326,11 +332,19 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
             }
         }
     } else {
+        // S = D*Gt - i Q*Gp ; T = -i Q*Gt - D*Gp (times 1/(l(l+1)) in the epilogue), with
+        // -i z = (Im z, -Re z): every B operand is one intermediate element times a sign.
         const int gt = item * gpi, gp = gt + 1, ge = gt + 2;
         const int eoD = 1 - par, eoQ = par;
         const long bt = (long)(gt >> 3) * mstride + mo + (gt & 7) * 2;
         const long bp = (long)(gp >> 3) * mstride + mo + (gp & 7) * 2;
         const long be = (long)(ge >> 3) * mstride + mo + (ge & 7) * 2;
+        const long offD = (comp == 0 ? bt : bp) + eoD * 16 + part;
+        const T sD = comp == 0 ? T(1) : T(-1);
+        const long offQ = (comp == 0 ? bp : bt) + eoQ * 16 + (1 - part);
+        const T sQ = part ? T(-1) : T(1);
+        const long offE = be + par * 16 + part;
+        const bool useE = has_ek && comp == 0;
         for (int p0 = 0; p0 < C.nhp; p0 += 4 * KCH) {
             T aD[KCH], aQ[KCH], aP[KCH], bD[KCH], bQ[KCH], bE[KCH];
 #pragma unroll
@@ -341,18 +355,9 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
                 aD[u] = (rv && pv) ? tD[abase + (long)pair * C.kp] : T(0);
                 aQ[u] = (rv && pv) ? tQ[abase + (long)pair * C.kp] : T(0);
                 aP[u] = (has_ek && rv && pv) ? tP[abase + (long)pair * C.kp] : T(0);
-                bD[u] = bQ[u] = bE[u] = T(0);
-                if (cv && pv) {
-                    // S = D*Gt - i Q*Gp ; T = -i Q*Gt - D*Gp   (times 1/(l(l+1)) in the epilogue)
-                    if (comp == 0) {
-                        bD[u] = four[bt + po + eoD * 16 + part];
-                        bQ[u] = part ? -four[bp + po + eoQ * 16] : four[bp + po + eoQ * 16 + 1];
-                        if (has_ek) bE[u] = four[be + po + par * 16 + part];
-                    } else {
-                        bD[u] = -four[bp + po + eoD * 16 + part];
-                        bQ[u] = part ? -four[bt + po + eoQ * 16] : four[bt + po + eoQ * 16 + 1];
-                    }
-                }
+                bD[u] = (cv && pv) ? sD * four[offD + po] : T(0);
+                bQ[u] = (cv && pv) ? sQ * four[offQ + po] : T(0);
+                bE[u] = (useE && cv && pv) ? four[offE + po] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < KCH; ++u) {
@@ -502,6 +507,11 @@ __host__ __device__ constexpr bool fft_fixed(int n)
 }
 __host__ __device__ constexpr int fft_fixed_rows(int n) { return n == 64 ? 16 : 8; }
 
+// LDS index with one padding element per 16: the Stockham stores of a radix-R pass
+// stride R elements across lanes, which without padding hits a few banks only.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+template <int ROWS, int N> constexpr int padded_len() { return ROWS * N + (ROWS * N) / 16 + 1; }
+
 template <int S, class T, int N, int ROWS, int P>
 __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__restrict__ dst,
                                       const cx<T> *__restrict__ tw)
@@ -520,9 +530,9 @@ __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__re
         const int rp = bi / NR, j = bi - rp * NR;
         const int k = j % Ns;
         cx<T> v[R];
-        const cx<T> *s = src + rp * N + j;
+        const int sb = rp * N + j;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = s[r * NR];
+        for (int r = 0; r < R; ++r) v[r] = src[pidx(sb + r * NR)];
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -532,9 +542,9 @@ __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__re
             }
         }
         dft<S, R>(v);
-        cx<T> *d = dst + rp * N + (j / Ns) * Ns * R + k;
+        const int db = rp * N + (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
+        for (int r = 0; r < R; ++r) dst[pidx(db + r * Ns)] = v[r];
     }
 }
 
@@ -643,15 +653,16 @@ template <class T, int N, int ROWS>
 __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, const T *__restrict__ four)
 {
     constexpr int NT = AER_FFT_THREADS;
+    constexpr int PL = padded_len<ROWS, N>();
     using T2 = cx<T>;
-    T2 *bufA = smem, *bufB = smem + ROWS * N, *tw = smem + 2 * ROWS * N;
+    T2 *bufA = smem, *bufB = smem + PL, *tw = smem + 2 * PL;
     const T2 *twg = (const T2 *)C.twN;
     for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
     const int M = C.T;
     const int nz = N - 2 * M - 1;
     for (int idx = threadIdx.x; idx < ROWS * nz; idx += NT) {
         const int rp = idx % ROWS, k = M + 1 + idx / ROWS;
-        bufA[rp * N + k] = T2{0, 0};
+        bufA[pidx(rp * N + k)] = T2{0, 0};
     }
     for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
         const int rp = idx % ROWS, m = idx / ROWS;
@@ -659,8 +670,8 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
         T2 zm{0, 0}, zc{0, 0};
         if (pair < C.nh && g * 8 + gfl < J.ngf)
             synth_pack(four + ((long)m * C.nhp + pair) * 32 + gfl * 2, m, zm, zc);
-        bufA[rp * N + m] = zm;
-        if (m > 0) bufA[rp * N + N - m] = zc;
+        bufA[pidx(rp * N + m)] = zm;
+        if (m > 0) bufA[pidx(rp * N + N - m)] = zc;
     }
     __syncthreads();
     const T2 *res = fft_block_fixed<+1, T, N, ROWS>(bufA, bufB, tw);
@@ -670,7 +681,7 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
         if (pair >= C.nh || gf >= J.ngf) continue;
         double sc;
         double *out = syn_out(J, gf, sc);
-        const T2 z = res[idx];
+        const T2 z = res[pidx(idx)];
         const int js = C.nlat - 1 - pair;
         out[(long)pair * N + i] = sc * (double)z.x;
         if (js != pair) out[(long)js * N + i] = sc * (double)z.y;
@@ -681,8 +692,9 @@ template <class T, int N, int ROWS>
 __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, T *__restrict__ four)
 {
     constexpr int NT = AER_FFT_THREADS;
+    constexpr int PL = padded_len<ROWS, N>();
     using T2 = cx<T>;
-    T2 *bufA = smem, *bufB = smem + ROWS * N, *tw = smem + 2 * ROWS * N;
+    T2 *bufA = smem, *bufB = smem + PL, *tw = smem + 2 * PL;
     const T2 *twg = (const T2 *)C.twN;
     for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
     for (int idx = threadIdx.x; idx < ROWS * N; idx += NT) {
@@ -694,7 +706,7 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
             const double *in = ana_in(J, gf, sc);
             z = T2{T(sc * in[(long)pair * N + i]), T(sc * in[(long)(C.nlat - 1 - pair) * N + i])};
         }
-        bufA[idx] = z;
+        bufA[pidx(idx)] = z;
     }
     __syncthreads();
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
@@ -705,7 +717,7 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
         if (pair >= C.nhp) continue;
         T *o = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
         if (pair < C.nh && g * 8 + gfl < J.ngf) {
-            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(C.wpair[pair]), o);
+            anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(C.wpair[pair]), o);
         } else {
             o[0] = o[1] = o[16] = o[17] = T(0);
         }

@@ -181,6 +181,8 @@ contains
         call aeros_sht_analysis_levels(ref, fld_b, out_b)
         call check(all(fld_b == fld_s), "level-batched synthesis is bit-identical to per-level", nfail)
         call check(all(out_b == out_s), "level-batched analysis is bit-identical to per-level", nfail)
+        write(*,"(a44,es12.3,es12.3)") "   batched vs per-level max |diff| ", &
+            maxval(abs(fld_b - fld_s)), maxval(abs(out_b - out_s))
         call aeros_sht_pool_end(pool)
         call aeros_sht_end(ref)
     end subroutine test_levels_and_pool

@@ -166,25 +166,33 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
     if (active) {
         const int nl = C.T - m + 1;
         const bool scalar = J.kind == SYN_SCALAR;
-        for (int seg = 0; seg < 8; ++seg) {
-            const double *src = nullptr, *fac = nullptr;
-            if (scalar) {
-                const int f = g * 8 + seg;
-                if (f < J.nitem) { src = J.sin0 + (long)f * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
-            } else {
-                const int it = g * 4 + (seg & 3);
-                if (it < J.nitem) {
-                    if (seg < 4) { src = J.sin0 + (long)it * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
-                    else if (J.kind == SYN_VECTOR) { src = J.sin1 + (long)it * J.sstride; fac = C.lfac + J.mode1 * (C.T + 1); }
+        // 64 degrees per pass; within a pass the 8 segments' loads are all issued
+        // before the first wait (registers, then LDS).
+        for (int n0 = 0; n0 < nl; n0 += 64) {
+            const int n = n0 + lane;
+            double re[8], im[8], fv[8];
+#pragma unroll
+            for (int seg = 0; seg < 8; ++seg) {
+                const double *src = nullptr, *fac = nullptr;
+                if (scalar) {
+                    const int f = g * 8 + seg;
+                    if (f < J.nitem) { src = J.sin0 + (long)f * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
+                } else {
+                    const int it = g * 4 + (seg & 3);
+                    if (it < J.nitem) {
+                        if (seg < 4) { src = J.sin0 + (long)it * J.sstride; fac = C.lfac + J.mode0 * (C.T + 1); }
+                        else if (J.kind == SYN_VECTOR) { src = J.sin1 + (long)it * J.sstride; fac = C.lfac + J.mode1 * (C.T + 1); }
+                    }
                 }
+                const bool v = src != nullptr && n < nl;
+                re[seg] = v ? src[2 * (lmb + n)] : 0.0;
+                im[seg] = v ? src[2 * (lmb + n) + 1] : 0.0;
+                fv[seg] = v ? fac[m + n] : 0.0;
             }
-            for (int n = lane; n < nl; n += 64) {
-                cx<T> v{T(0), T(0)};
-                if (src) {
-                    const double f = fac[m + n];
-                    v = cx<T>{T(f * src[2 * (lmb + n)]), T(f * src[2 * (lmb + n) + 1])};
-                }
-                sb[seg * A.kmax + n] = v;
+            if (n < nl) {
+#pragma unroll
+                for (int seg = 0; seg < 8; ++seg)
+                    sb[seg * A.kmax + n] = cx<T>{T(fv[seg] * re[seg]), T(fv[seg] * im[seg])};
             }
         }
     }
@@ -622,9 +630,8 @@ __device__ __forceinline__ const double *ana_in(const Job &J, int gf, double &sc
 // Z_m = F_N + i F_S for 0 < m <= M, Z_{N-m} = conj(F_N) + i conj(F_S), Z_0 = Re F_N + i Re F_S,
 // with F_N = E + O, F_S = E - O read from the intermediate.
 template <class T>
-__device__ __forceinline__ void synth_pack(const T *__restrict__ b, int m, cx<T> &zm, cx<T> &zc)
+__device__ __forceinline__ void synth_pack(T er, T ei, T or_, T oi, int m, cx<T> &zm, cx<T> &zc)
 {
-    const T er = b[0], ei = b[1], or_ = b[16], oi = b[17];
     const T nr = er + or_, ni = ei + oi, sr = er - or_, si = ei - oi;
     if (m == 0) {
         zm = cx<T>{nr, sr};
@@ -657,21 +664,50 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
     using T2 = cx<T>;
     T2 *bufA = smem, *bufB = smem + PL, *tw = smem + 2 * PL;
     const T2 *twg = (const T2 *)C.twN;
-    for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
     const int M = C.T;
+    // Every global load of the block is issued before the first wait: the
+    // intermediate values (T + 1 <= N/2 since the grid has nlon > 2T) and the
+    // twiddles go to registers, then to LDS.
+    constexpr int NLD = (ROWS * (N / 2) + NT - 1) / NT, NTW = (N + NT - 1) / NT;
+    T er[NLD], ei[NLD], orr[NLD], oi[NLD];
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+        const int idx = threadIdx.x + u * NT;
+        const int rp = idx % ROWS, m = idx / ROWS;
+        const int gfl = rp & 7, pair = p0 + (rp >> 3);
+        const bool v = m <= M && pair < C.nh && g * 8 + gfl < J.ngf;
+        const T *b = four + ((long)(v ? m : 0) * C.nhp + (v ? pair : 0)) * 32 + gfl * 2;
+        er[u] = v ? b[0] : T(0);
+        ei[u] = v ? b[1] : T(0);
+        orr[u] = v ? b[16] : T(0);
+        oi[u] = v ? b[17] : T(0);
+    }
+    T2 twv[NTW];
+#pragma unroll
+    for (int u = 0; u < NTW; ++u) {
+        const int k = threadIdx.x + u * NT;
+        twv[u] = k < N ? twg[k] : T2{0, 0};
+    }
     const int nz = N - 2 * M - 1;
     for (int idx = threadIdx.x; idx < ROWS * nz; idx += NT) {
         const int rp = idx % ROWS, k = M + 1 + idx / ROWS;
         bufA[pidx(rp * N + k)] = T2{0, 0};
     }
-    for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+        const int idx = threadIdx.x + u * NT;
         const int rp = idx % ROWS, m = idx / ROWS;
-        const int gfl = rp & 7, pair = p0 + (rp >> 3);
-        T2 zm{0, 0}, zc{0, 0};
-        if (pair < C.nh && g * 8 + gfl < J.ngf)
-            synth_pack(four + ((long)m * C.nhp + pair) * 32 + gfl * 2, m, zm, zc);
-        bufA[pidx(rp * N + m)] = zm;
-        if (m > 0) bufA[pidx(rp * N + N - m)] = zc;
+        if (m <= M) {
+            T2 zm, zc;
+            synth_pack(er[u], ei[u], orr[u], oi[u], m, zm, zc);
+            bufA[pidx(rp * N + m)] = zm;
+            if (m > 0) bufA[pidx(rp * N + N - m)] = zc;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NTW; ++u) {
+        const int k = threadIdx.x + u * NT;
+        if (k < N) tw[k] = twv[u];
     }
     __syncthreads();
     const T2 *res = fft_block_fixed<+1, T, N, ROWS>(bufA, bufB, tw);
@@ -696,17 +732,36 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     using T2 = cx<T>;
     T2 *bufA = smem, *bufB = smem + PL, *tw = smem + 2 * PL;
     const T2 *twg = (const T2 *)C.twN;
-    for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
-    for (int idx = threadIdx.x; idx < ROWS * N; idx += NT) {
+    // All grid loads of the block are issued before the first wait (registers, then LDS).
+    static_assert((ROWS * N) % NT == 0, "fixed FFT rows must tile the block");
+    constexpr int NLD = ROWS * N / NT, NTW = (N + NT - 1) / NT;
+    double zn[NLD], zs[NLD], scv[NLD];
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+        const int idx = threadIdx.x + u * NT;
         const int rp = idx / N, i = idx - rp * N;
         const int gfl = rp & 7, pair = p0 + (rp >> 3), gf = g * 8 + gfl;
-        T2 z{0, 0};
-        if (pair < C.nh && gf < J.ngf) {
-            double sc;
-            const double *in = ana_in(J, gf, sc);
-            z = T2{T(sc * in[(long)pair * N + i]), T(sc * in[(long)(C.nlat - 1 - pair) * N + i])};
-        }
-        bufA[pidx(idx)] = z;
+        const bool v = pair < C.nh && gf < J.ngf;
+        double sc = 0.0;
+        const double *in = ana_in(J, v ? gf : 0, sc);
+        scv[u] = v ? sc : 0.0;
+        const int pn = v ? pair : 0;
+        zn[u] = v ? in[(long)pn * N + i] : 0.0;
+        zs[u] = v ? in[(long)(C.nlat - 1 - pn) * N + i] : 0.0;
+    }
+    T2 twv[NTW];
+#pragma unroll
+    for (int u = 0; u < NTW; ++u) {
+        const int k = threadIdx.x + u * NT;
+        twv[u] = k < N ? twg[k] : T2{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < NLD; ++u)
+        bufA[pidx(threadIdx.x + u * NT)] = T2{T(scv[u] * zn[u]), T(scv[u] * zs[u])};
+#pragma unroll
+    for (int u = 0; u < NTW; ++u) {
+        const int k = threadIdx.x + u * NT;
+        if (k < N) tw[k] = twv[u];
     }
     __syncthreads();
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
@@ -741,7 +796,8 @@ __device__ __forceinline__ void fft_synth_generic(const Job &J, const CfgDev &C,
             else if (k >= N - M) m = N - k;
             if (m >= 0) {
                 T2 zm, zc;
-                synth_pack(four + ((long)m * C.nhp + pair) * 32 + gfl * 2, m, zm, zc);
+                const T *b = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
+                synth_pack(b[0], b[1], b[16], b[17], m, zm, zc);
                 z = (k <= M) ? zm : zc;
             }
         }

@@ -74,13 +74,14 @@ class AerJob(ctypes.Structure):
                 ("in_stride", _c_long), ("out_stride", _c_long),
                 ("mode0", _c_int), ("mode1", _c_int), ("mode2", _c_int),
                 ("scale0", _c_double), ("scale1", _c_double),
-                ("accumulate", _c_int)]
+                ("accumulate", _c_int), ("prec", _c_int)]
 
 
 AER_JOB_SYNTH_SCALAR, AER_JOB_SYNTH_VECTOR, AER_JOB_SYNTH_GRAD, AER_JOB_ANAL_SCALAR, AER_JOB_ANAL_VECTOR = range(5)
 (AER_MODE_ONE, AER_MODE_S_FROM_DIV, AER_MODE_T_FROM_VOR, AER_MODE_INV_LL1, AER_MODE_DIV_FROM_S,
  AER_MODE_VOR_FROM_T, AER_MODE_NEG_LAP, AER_MODE_ZERO) = range(8)
 AER_MAX_JOBS = 8
+AER_JOB_PREC_HANDLE, AER_JOB_PREC_FP64, AER_JOB_PREC_FP32 = 0, 1, 2
 SIGNATURES["aer_run_jobs_d"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob)])
 SIGNATURES["aer_plan_create"] = (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(AerJob), ctypes.POINTER(_vp)])
 SIGNATURES["aer_plan_run"] = (_c_int, [_vp, _vp])

@@ -303,36 +303,43 @@ struct JobSpec {
     int mode0 = aer::M_ONE, mode1 = aer::M_ONE, mode2 = aer::M_ZERO;
     double gsc0 = 1.0, gsc1 = 1.0;
     int accumulate = 0;
+    int prec = -1;          // compute type: 0 = FP64, 1 = FP32; -1 = the handle's (resolved at launch)
 };
 
 // Job table of one direction: jobs on handles hs[k] (one truncation each), their
 // task ranges in the Legendre and FFT launches, the intermediate size and the FFT
-// kernel's dynamic LDS.
+// kernel's dynamic LDS. Each job's Legendre range is padded to whole 4-wave blocks.
 struct Table {
     std::vector<aer::Job> jobs;
     int ltask = 0, ftask = 0;
-    long four_elems = 0;
+    size_t four_bytes = 0;
     size_t lds = 0;
     bool generic = false;   // some job needs the runtime-plan FFT path
     int kmax = 1;           // max T+1 over the jobs
+    int pm = aer::PM_FP64;  // kernel precision mode
+    size_t cxmax = 16;      // largest complex element size among the jobs
 };
 
-template <int PREC>
 void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
 {
-    using T = typename aer::Prec<PREC>::T;
     B = Table{};
+    bool any64 = false, any32 = false;
+    size_t cxmax = 0;
     for (int k = 0; k < njob; ++k) {
         const JobSpec &s = specs[k];
         if (s.nitem <= 0) continue;
         const aer_sht *h = hs[k];
-        const aer::CfgDev C = cfg_dev(h, PREC);
+        const aer::CfgDev C = cfg_dev(h, s.prec ? AER_PREC_FP32 : AER_PREC_FP64);
+        const size_t cxb = s.prec ? 2 * sizeof(float) : 2 * sizeof(double);
+        (s.prec ? any32 : any64) = true;
+        cxmax = std::max(cxmax, cxb);
         const int ntpp = ((C.T + 2) / 2 + 15) / 16;
         const int nsub = 8 / h->fb;
         aer::Job J{};
         J.c = C;
         J.kind = s.kind;
         J.nitem = s.nitem;
+        J.prec = s.prec;
         switch (s.kind) {
             case aer::SYN_SCALAR: case aer::ANA_SCALAR: J.ngf = s.nitem; break;
             case aer::SYN_VECTOR: case aer::SYN_GRAD: J.ngf = 2 * s.nitem; break;
@@ -350,18 +357,13 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         J.mode0 = s.mode0; J.mode1 = s.mode1; J.mode2 = s.mode2;
         J.gsc0 = s.gsc0; J.gsc1 = s.gsc1;
         J.accumulate = s.accumulate;
-        J.four_off = B.four_elems;
-        B.four_elems += (long)J.ngroup * (C.T + 1) * C.nhp * 32;
+        J.four_off = (long)B.four_bytes;
+        B.four_bytes += ((size_t)J.ngroup * (C.T + 1) * C.nhp * 32 * (cxb / 2) + 255) / 256 * 256;
         J.ltask0 = B.ltask;
         J.ftask0 = B.ftask;
-        if (synth) {
-            B.ltask += J.lgroups * (C.T + 1) * C.npt;
-            B.ftask += J.ngroup * nsub * ((C.nh + C.pb - 1) / C.pb);
-        } else {
-            B.ltask += J.lgroups * (C.T + 1) * 2 * ntpp;
-            B.ftask += J.ngroup * nsub * ((C.nhp + C.pb - 1) / C.pb);
-        }
-        const size_t cxb = 2 * sizeof(T);
+        J.ltasks = synth ? J.lgroups * (C.T + 1) * C.npt : J.lgroups * (C.T + 1) * 2 * ntpp;
+        B.ltask += (J.ltasks + 3) / 4 * 4;
+        B.ftask += J.ngroup * nsub * (((synth ? C.nh : C.nhp) + C.pb - 1) / C.pb);
         const size_t rn = (size_t)aer::fft_fixed_rows(h->nlon) * h->nlon;
         const size_t lds = aer::fft_fixed(h->nlon)
                                ? (2 * (rn + rn / 16 + 1) + h->nlon) * cxb
@@ -371,19 +373,19 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         B.kmax = std::max(B.kmax, C.T + 1);
         B.jobs.push_back(J);
     }
+    B.pm = any64 && any32 ? aer::PM_MIXED : (any32 ? aer::PM_FP32 : aer::PM_FP64);
+    B.cxmax = cxmax ? cxmax : 16;
 }
 
-// The two launches of one direction. th: the handle whose timing hook records them.
-template <int PREC>
-int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
-                 const int *tj_leg = nullptr, const int *tj_fft = nullptr)
+template <int PM, bool PLAN>
+int launch_table_pm(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
+                     const int *tj_leg, const int *tj_fft)
 {
-    using T = typename aer::Prec<PREC>::T;
     aer::LaunchArgs LL = L, LF = L;
     LL.task_job = tj_leg;
     LF.task_job = tj_fft;
     LL.kmax = LF.kmax = B.kmax;
-    const size_t lds_leg = synth ? (size_t)4 * 8 * B.kmax * 2 * sizeof(T) : 0;
+    const size_t lds_leg = synth ? (size_t)4 * 8 * B.kmax * B.cxmax : 0;
     LL.ntask = B.ltask;
     LF.ntask = B.ftask;
     const int lb = (B.ltask + 3) / 4, fb = B.ftask;
@@ -391,61 +393,81 @@ int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Ta
     int rc;
     if (synth) {
         rc = timed_launch(th, 0, [&] {
-            hipLaunchKernelGGL(aer::k_leg_synth<PREC>, dim3(lb), dim3(256), lds_leg, st, LL);
+            hipLaunchKernelGGL((aer::k_leg_synth<PM, PLAN>), dim3(lb), dim3(256), lds_leg, st, LL);
         });
         if (rc) return rc;
         return timed_launch(th, 1, [&] {
             if (B.generic)
-                hipLaunchKernelGGL((aer::k_fft_synth<PREC, true>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+                hipLaunchKernelGGL((aer::k_fft_synth<PM, true, PLAN>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
             else
-                hipLaunchKernelGGL((aer::k_fft_synth<PREC, false>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+                hipLaunchKernelGGL((aer::k_fft_synth<PM, false, PLAN>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
         });
     }
     rc = timed_launch(th, 2, [&] {
         if (B.generic)
-            hipLaunchKernelGGL((aer::k_fft_anal<PREC, true>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+            hipLaunchKernelGGL((aer::k_fft_anal<PM, true, PLAN>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
         else
-            hipLaunchKernelGGL((aer::k_fft_anal<PREC, false>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
+            hipLaunchKernelGGL((aer::k_fft_anal<PM, false, PLAN>), dim3(fb), dim3(AER_FFT_THREADS), lds, st, LF);
     });
     if (rc) return rc;
     return timed_launch(th, 3, [&] {
-        hipLaunchKernelGGL(aer::k_leg_anal<PREC>, dim3(lb), dim3(256), 0, st, LL);
+        hipLaunchKernelGGL((aer::k_leg_anal<PM, PLAN>), dim3(lb), dim3(256), 0, st, LL);
     });
 }
 
-template <int PREC>
-int launch_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
+// The two launches of one direction. th: the handle whose timing hook records them.
+int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
+                 const int *tj_leg = nullptr, const int *tj_fft = nullptr)
 {
-    using T = typename aer::Prec<PREC>::T;
+    const bool plan = L.djobs != nullptr;
+    switch (B.pm) {
+        case aer::PM_FP64:
+            return plan ? launch_table_pm<aer::PM_FP64, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                        : launch_table_pm<aer::PM_FP64, false>(th, st, L, B, synth, tj_leg, tj_fft);
+        case aer::PM_FP32:
+            return plan ? launch_table_pm<aer::PM_FP32, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                        : launch_table_pm<aer::PM_FP32, false>(th, st, L, B, synth, tj_leg, tj_fft);
+        default:
+            return plan ? launch_table_pm<aer::PM_MIXED, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                        : launch_table_pm<aer::PM_MIXED, false>(th, st, L, B, synth, tj_leg, tj_fft);
+    }
+}
+
+// Resolve a job's compute type (its own, else the handle's setting) and make sure
+// the FP32 tables exist when it needs them.
+int resolve_prec(aer_sht *h, JobSpec &s)
+{
+    if (s.prec < 0) {
+        if (h->prec != AER_PREC_FP64 && h->prec != AER_PREC_FP32)
+            return fail(AER_EINVAL, "precision not available in this build");
+        s.prec = h->prec == AER_PREC_FP32 ? 1 : 0;
+    }
+    if (s.prec) return ensure_tables32(h);
+    return AER_OK;
+}
+
+int run_jobs(aer_sht *h, JobSpec *specs, int njob, bool synth)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
+    if (njob > AER_MAXJOB) return fail(AER_EINVAL, "too many jobs in one launch");
     aer_sht *hs[AER_MAXJOB];
-    for (int k = 0; k < njob; ++k) hs[k] = h;
+    for (int k = 0; k < njob; ++k) {
+        hs[k] = h;
+        int rc = resolve_prec(h, specs[k]);
+        if (rc) return rc;
+    }
     Table B;
-    build_table<PREC>(hs, specs, njob, synth, B);
+    build_table(hs, specs, njob, synth, B);
     if (B.jobs.empty()) return AER_OK;
-    int rc = h->four.ensure((size_t)B.four_elems * sizeof(T));
+    int rc = h->four.ensure(B.four_bytes);
     if (rc) return rc;
     aer::LaunchArgs L{};
     L.njob = (int)B.jobs.size();
     L.four = h->four.p;
     L.djobs = nullptr;
     for (int k = 0; k < L.njob; ++k) L.job[k] = B.jobs[k];
-    return launch_table<PREC>(h, h->stream, L, B, synth);
-}
-
-int run_jobs(aer_sht *h, const JobSpec *specs, int njob, bool synth)
-{
-    if (!h) return fail(AER_EINVAL, "null handle");
-    if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
-    if (njob > AER_MAXJOB) return fail(AER_EINVAL, "too many jobs in one launch");
-    switch (h->prec) {
-        case AER_PREC_FP64: return launch_jobs<0>(h, specs, njob, synth);
-        case AER_PREC_FP32: {
-            int rc = ensure_tables32(h);
-            if (rc) return rc;
-            return launch_jobs<1>(h, specs, njob, synth);
-        }
-        default: return fail(AER_EINVAL, "precision not available in this build");
-    }
+    return launch_table(h, h->stream, L, B, synth);
 }
 
 long dense_s(const aer_sht *h) { return 2L * h->nlm; }
@@ -785,13 +807,18 @@ int job_to_spec(const aer_sht *h, const aer_job &j, JobSpec &s)
     s.gsc0 = j.scale0 != 0.0 ? j.scale0 : 1.0;
     s.gsc1 = j.scale1 != 0.0 ? j.scale1 : 1.0;
     s.accumulate = j.accumulate;
+    switch (j.prec) {
+        case AER_JOB_PREC_HANDLE: s.prec = -1; break;
+        case AER_JOB_PREC_FP64: s.prec = 0; break;
+        case AER_JOB_PREC_FP32: s.prec = 1; break;
+        default: return fail(AER_EINVAL, "job: unknown precision");
+    }
     return AER_OK;
 }
 
 }  // namespace
 
 struct aer_plan {
-    int prec = AER_PREC_FP64;
     aer_sht *th = nullptr;           // timing hook owner (first job's handle)
     Table syn, ana;
     DevBuf djob_syn, djob_ana, four;
@@ -800,10 +827,8 @@ struct aer_plan {
 
 namespace {
 
-template <int PREC>
 int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &specs)
 {
-    using T = typename aer::Prec<PREC>::T;
     std::vector<aer_sht *> hsy, han;
     std::vector<JobSpec> ssy, san;
     for (size_t k = 0; k < specs.size(); ++k) {
@@ -811,8 +836,8 @@ int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &sp
         (synth ? hsy : han).push_back(hs[k]);
         (synth ? ssy : san).push_back(specs[k]);
     }
-    build_table<PREC>(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
-    build_table<PREC>(han.data(), san.data(), (int)san.size(), false, p->ana);
+    build_table(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
+    build_table(han.data(), san.data(), (int)san.size(), false, p->ana);
     int rc;
     if (!p->syn.jobs.empty() && (rc = upload(p->djob_syn, p->syn.jobs.data(), p->syn.jobs.size() * sizeof(aer::Job))))
         return rc;
@@ -834,11 +859,9 @@ int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &sp
         if ((rc = upload(bl, tl.data(), std::max<size_t>(tl.size(), 1) * sizeof(int)))) return rc;
         if ((rc = upload(bf, tf.data(), std::max<size_t>(tf.size(), 1) * sizeof(int)))) return rc;
     }
-    const long four = std::max(p->syn.four_elems, p->ana.four_elems);
-    return p->four.ensure((size_t)std::max(four, 1L) * sizeof(T));
+    return p->four.ensure(std::max<size_t>(std::max(p->syn.four_bytes, p->ana.four_bytes), 256));
 }
 
-template <int PREC>
 int plan_launch(aer_plan *p, hipStream_t st)
 {
     aer::LaunchArgs L{};
@@ -847,13 +870,13 @@ int plan_launch(aer_plan *p, hipStream_t st)
     if (!p->syn.jobs.empty()) {
         L.njob = (int)p->syn.jobs.size();
         L.djobs = (const aer::Job *)p->djob_syn.p;
-        if ((rc = launch_table<PREC>(p->th, st, L, p->syn, true, (const int *)p->tj[0].p, (const int *)p->tj[1].p)))
+        if ((rc = launch_table(p->th, st, L, p->syn, true, (const int *)p->tj[0].p, (const int *)p->tj[1].p)))
             return rc;
     }
     if (!p->ana.jobs.empty()) {
         L.njob = (int)p->ana.jobs.size();
         L.djobs = (const aer::Job *)p->djob_ana.p;
-        if ((rc = launch_table<PREC>(p->th, st, L, p->ana, false, (const int *)p->tj[3].p, (const int *)p->tj[2].p)))
+        if ((rc = launch_table(p->th, st, L, p->ana, false, (const int *)p->tj[3].p, (const int *)p->tj[2].p)))
             return rc;
     }
     return AER_OK;
@@ -870,25 +893,18 @@ int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_pl
     if (njob <= 0 || !cfgs || !jobs) return fail(AER_EINVAL, "aer_plan_create: empty job list");
     std::vector<aer_sht *> hs(njob);
     std::vector<JobSpec> specs(njob);
-    int prec = -1, rc;
+    int rc;
     for (int k = 0; k < njob; ++k) {
         aer_sht *h = cfgs[k];
         if (!h) return fail(AER_EINVAL, "aer_plan_create: null handle");
         if (!h->grid_set) return fail(AER_ESTATE, "aer_plan_create: handle without a grid");
-        if (prec < 0) prec = h->prec;
-        if (h->prec != prec) return fail(AER_EINVAL, "aer_plan_create: handles differ in precision");
         if ((rc = job_to_spec(h, jobs[k], specs[k]))) return rc;
-        if (h->prec == AER_PREC_FP32 && (rc = ensure_tables32(h))) return rc;
+        if ((rc = resolve_prec(h, specs[k]))) return rc;
         hs[k] = h;
     }
     aer_plan *p = new aer_plan();
-    p->prec = prec;
     p->th = hs[0];
-    switch (prec) {
-        case AER_PREC_FP64: rc = plan_build<0>(p, hs, specs); break;
-        case AER_PREC_FP32: rc = plan_build<1>(p, hs, specs); break;
-        default: rc = fail(AER_EINVAL, "aer_plan_create: precision not available");
-    }
+    rc = plan_build(p, hs, specs);
     if (rc) {
         aer_plan_destroy(p);
         return rc;
@@ -900,11 +916,7 @@ int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_pl
 int aer_plan_run(aer_plan_t p, void *stream)
 {
     if (!p) return fail(AER_EINVAL, "aer_plan_run: null plan");
-    switch (p->prec) {
-        case AER_PREC_FP64: return plan_launch<0>(p, (hipStream_t)stream);
-        case AER_PREC_FP32: return plan_launch<1>(p, (hipStream_t)stream);
-        default: return fail(AER_EINVAL, "aer_plan_run: precision not available");
-    }
+    return plan_launch(p, (hipStream_t)stream);
 }
 
 int aer_plan_destroy(aer_plan_t p)

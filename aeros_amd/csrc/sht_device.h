@@ -72,8 +72,14 @@ struct Job {
     int mode0, mode1, mode2;
     double gsc0, gsc1;                // grid-side scale of component 0 / 1
     int accumulate;
-    long four_off;                    // element offset of the job's intermediate region
+    int prec;                         // compute type of this job: 0 = FP64, 1 = FP32
+    int ltasks;                       // Legendre tasks of the job (its range is padded to whole blocks)
+    long four_off;                    // byte offset of the job's intermediate region
 };
+
+// Kernel precision modes: every job FP64, every job FP32, or per job (the kernel
+// branches on Job::prec; a block never spans two jobs, so the branch is uniform).
+enum PrecMode { PM_FP64 = 0, PM_FP32 = 1, PM_MIXED = 2 };
 
 // Job table of one launch: inline in the kernel arguments (up to AER_MAXJOB jobs),
 // or in device memory (djobs != nullptr, any number of jobs; compiled plans).
@@ -115,15 +121,23 @@ template <> struct Prec<1> {
 // The job a task belongs to: the last job whose first task is <= task. The task
 // index is uniform over a wave, so the job index is too (readfirstlane lets the
 // compiler read the job's fields with scalar loads).
+// PLAN (a template parameter, so that one kernel never mixes a pointer into the
+// kernel arguments with one into device memory): jobs in a device table.
+template <bool PLAN>
 __device__ __forceinline__ const Job &job_of(const LaunchArgs &A, int task, bool legendre)
 {
-    if (A.djobs) {
+    if constexpr (PLAN) {
         const int jb = __builtin_amdgcn_readfirstlane(A.task_job[task]);
         return A.djobs[jb];
     }
+    // Inline table: indexed straight in the kernel-argument segment (LaunchArgs is
+    // the only kernel argument, at offset 0), not through the by-value parameter,
+    // whose dynamically indexed use would keep a private copy of all of it.
+    const LaunchArgs *K = (const LaunchArgs *)__builtin_amdgcn_kernarg_segment_ptr();
     int jb = 0;
-    while (jb + 1 < A.njob && task >= (legendre ? A.job[jb + 1].ltask0 : A.job[jb + 1].ftask0)) ++jb;
-    return A.job[jb];
+    while (jb + 1 < K->njob && task >= (legendre ? K->job[jb + 1].ltask0 : K->job[jb + 1].ftask0)) ++jb;
+    jb = __builtin_amdgcn_readfirstlane(jb);
+    return K->job[jb];
 }
 
 // 0-based index of (l = m, m) in the m-major, l-fastest layout: the orders m' < m
@@ -145,17 +159,13 @@ constexpr int KCH = KCH_A;
 // B operands are then read from LDS. Segment s of a wave holds field g*8+s (scalar
 // jobs) or S (s < 4) / T (s >= 4) of item g*4 + s%4 (vector and gradient jobs).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
+__device__ __forceinline__ void leg_synth_body(const LaunchArgs &A, const Job &J, int task, bool active,
+                                               int wave, int lane, char *smem_raw)
 {
     using P = Prec<PREC>;
     using T = typename P::T;
     using Acc = typename P::Acc;
-    extern __shared__ char smem_raw[];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int task = blockIdx.x * 4 + wave;
-    const bool active = task < A.ntask;
     cx<T> *sb = (cx<T> *)smem_raw + (long)wave * 8 * A.kmax;
-    const Job &J = job_of(A, active ? task : 0, true);
     const CfgDev &C = J.c;
     int t = task - J.ltask0;
     const int g = t % J.lgroups;
@@ -269,7 +279,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
             }
         }
     }
-    T *four = (T *)A.four + J.four_off;
+    T *four = (T *)((char *)A.four + J.four_off);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int pair = pt * 16 + P::row(kq, r);
@@ -279,20 +289,35 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
     }
 }
 
+// The block's 4 waves: tasks blockIdx*4 .. +3, all of one job (job task ranges are
+// padded to multiples of 4), so the precision branch and the staging barrier are
+// block-uniform.
+template <int PM, bool PLAN>
+__global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
+{
+    extern __shared__ char smem_raw[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int task = blockIdx.x * 4 + wave;
+    const Job &J = job_of<PLAN>(A, task < A.ntask ? task : A.ntask - 1, true);
+    const bool active = task < A.ntask && task - J.ltask0 < J.ltasks;
+    if constexpr (PM == PM_MIXED) {
+        if (J.prec) leg_synth_body<1>(A, J, task, active, wave, lane, smem_raw);
+        else leg_synth_body<0>(A, J, task, active, wave, lane, smem_raw);
+    } else {
+        leg_synth_body<PM>(A, J, task, active, wave, lane, smem_raw);
+    }
+}
+
 // ------------------------------------------------------------------ Legendre analysis
 // One wavefront per task (job, column group, m, parity x tile of 16 degrees).
 // Rows = degrees l of one parity, K = latitude pairs, columns = 8 complex outputs
 // (scalar) or 4 (S, T) pairs (vector).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
+__device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J, int task, int lane)
 {
     using P = Prec<PREC>;
     using T = typename P::T;
     using Acc = typename P::Acc;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int task = blockIdx.x * 4 + wave;
-    if (task >= A.ntask) return;
-    const Job &J = job_of(A, task, true);
     const CfgDev &C = J.c;
     const int ntpp = ((C.T + 2) / 2 + 15) / 16;  // tiles per parity
     int t = task - J.ltask0;
@@ -312,7 +337,7 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     const T *__restrict__ tP = (const T *)C.tabPT;
     const T *__restrict__ tD = (const T *)C.tabDT;
     const T *__restrict__ tQ = (const T *)C.tabQT;
-    const T *__restrict__ four = (const T *)A.four + J.four_off;
+    const T *__restrict__ four = (const T *)((const char *)A.four + J.four_off);
     const long mstride = (long)(C.T + 1) * C.nhp * 32;  // one intermediate group
     const long mo = (long)m * C.nhp * 32;
     Acc acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
@@ -392,6 +417,22 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
         if (has_ek && comp == 0) v += fe[l] * (double)acc2[r];
         if (J.accumulate) out[2 * lm + part] += v;
         else out[2 * lm + part] = v;
+    }
+}
+
+template <int PM, bool PLAN>
+__global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int task = blockIdx.x * 4 + wave;
+    if (task >= A.ntask) return;
+    const Job &J = job_of<PLAN>(A, task, true);
+    if (task - J.ltask0 >= J.ltasks) return;
+    if constexpr (PM == PM_MIXED) {
+        if (J.prec) leg_anal_body<1>(A, J, task, lane);
+        else leg_anal_body<0>(A, J, task, lane);
+    } else {
+        leg_anal_body<PM>(A, J, task, lane);
     }
 }
 
@@ -735,7 +776,7 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     // All grid loads of the block are issued before the first wait (registers, then LDS).
     static_assert((ROWS * N) % NT == 0, "fixed FFT rows must tile the block");
     constexpr int NLD = ROWS * N / NT, NTW = (N + NT - 1) / NT;
-    double zn[NLD], zs[NLD], scv[NLD];
+    T2 z[NLD];
 #pragma unroll
     for (int u = 0; u < NLD; ++u) {
         const int idx = threadIdx.x + u * NT;
@@ -744,10 +785,8 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
         const bool v = pair < C.nh && gf < J.ngf;
         double sc = 0.0;
         const double *in = ana_in(J, v ? gf : 0, sc);
-        scv[u] = v ? sc : 0.0;
         const int pn = v ? pair : 0;
-        zn[u] = v ? in[(long)pn * N + i] : 0.0;
-        zs[u] = v ? in[(long)(C.nlat - 1 - pn) * N + i] : 0.0;
+        z[u] = v ? T2{T(sc * in[(long)pn * N + i]), T(sc * in[(long)(C.nlat - 1 - pn) * N + i])} : T2{0, 0};
     }
     T2 twv[NTW];
 #pragma unroll
@@ -756,23 +795,25 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
         twv[u] = k < N ? twg[k] : T2{0, 0};
     }
 #pragma unroll
-    for (int u = 0; u < NLD; ++u)
-        bufA[pidx(threadIdx.x + u * NT)] = T2{T(scv[u] * zn[u]), T(scv[u] * zs[u])};
+    for (int u = 0; u < NLD; ++u) bufA[pidx(threadIdx.x + u * NT)] = z[u];
 #pragma unroll
     for (int u = 0; u < NTW; ++u) {
         const int k = threadIdx.x + u * NT;
         if (k < N) tw[k] = twv[u];
     }
     __syncthreads();
+    // job fields read before the global stores below (kernel-argument memory is
+    // then never re-read after a store that might alias it)
+    const int M = C.T, nh = C.nh, nhp = C.nhp, ngf = J.ngf;
+    const double *wpair = C.wpair;
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
-    const int M = C.T;
     for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
         const int rp = idx % ROWS, m = idx / ROWS;
         const int gfl = rp & 7, pair = p0 + (rp >> 3);
-        if (pair >= C.nhp) continue;
-        T *o = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
-        if (pair < C.nh && g * 8 + gfl < J.ngf) {
-            anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(C.wpair[pair]), o);
+        if (pair >= nhp) continue;
+        T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
+        if (pair < nh && g * 8 + gfl < ngf) {
+            anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(wpair[pair]), o);
         } else {
             o[0] = o[1] = o[16] = o[17] = T(0);
         }
@@ -837,22 +878,25 @@ __device__ __forceinline__ void fft_anal_generic(const Job &J, const CfgDev &C, 
     }
     __syncthreads();
     const T2 *res = fft_rows<-1, T>(bufA, bufB, rows, C);
+    const int nh = C.nh, nhp = C.nhp, ngf = J.ngf, fbk = C.fb, pbk = C.pb;
+    const double *wpair = C.wpair;
     for (int idx = threadIdx.x; idx < rows * (M + 1); idx += blockDim.x) {
         const int rp = idx / (M + 1), m = idx - rp * (M + 1);
-        const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb;
-        if (pair >= C.nhp) continue;
-        T *o = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
-        if (pair < C.nh && g * 8 + gfl < J.ngf)
-            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(C.wpair[pair]), o);
+        const int gfl = sub * fbk + rp / pbk, pair = p0 + rp % pbk;
+        if (pair >= nhp) continue;
+        T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
+        if (pair < nh && g * 8 + gfl < ngf)
+            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(wpair[pair]), o);
         else
             o[0] = o[1] = o[16] = o[17] = T(0);
     }
 }
 
 // Block -> (job, intermediate group, field subgroup, first latitude pair).
+template <bool PLAN>
 __device__ __forceinline__ const Job &fft_task(const LaunchArgs &A, int task, int &g, int &sub, int &p0)
 {
-    const Job &J = job_of(A, task, false);
+    const Job &J = job_of<PLAN>(A, task, false);
     const int nsub = 8 / J.c.fb;
     int t = task - J.ftask0;
     g = t % J.ngroup;
@@ -877,17 +921,12 @@ __device__ __forceinline__ const Job &fft_task(const LaunchArgs &A, int task, in
 // GENERIC = false: only the compile-time lengths (no scratch); true: also the
 // runtime-plan path (the host picks it when some job has another grid length).
 template <int PREC, bool GENERIC>
-__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_synth(const LaunchArgs A)
+__device__ __forceinline__ void fft_synth_body(const Job &J, int g, int sub, int p0, char *smem_raw, const void *four_base)
 {
     using T = typename Prec<PREC>::T;
-    extern __shared__ char smem_raw[];
-    const int task = blockIdx.x;
-    if (task >= A.ntask) return;
-    int g, sub, p0;
-    const Job &J = fft_task(A, task, g, sub, p0);
     const CfgDev &C = J.c;
     cx<T> *smem = (cx<T> *)smem_raw;
-    const T *four = (const T *)A.four + J.four_off + (long)g * (C.T + 1) * C.nhp * 32;
+    const T *four = (const T *)((const char *)four_base + J.four_off) + (long)g * (C.T + 1) * C.nhp * 32;
     if (fft_fixed(C.nlon)) {
         AER_FFT_DISPATCH(fft_synth, J, C, g, p0, smem, four)
     } else if constexpr (GENERIC) {
@@ -896,21 +935,48 @@ __global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_synth(const LaunchArgs 
 }
 
 template <int PREC, bool GENERIC>
-__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_anal(const LaunchArgs A)
+__device__ __forceinline__ void fft_anal_body(const Job &J, int g, int sub, int p0, char *smem_raw, void *four_base)
 {
     using T = typename Prec<PREC>::T;
-    extern __shared__ char smem_raw[];
-    const int task = blockIdx.x;
-    if (task >= A.ntask) return;
-    int g, sub, p0;
-    const Job &J = fft_task(A, task, g, sub, p0);
     const CfgDev &C = J.c;
     cx<T> *smem = (cx<T> *)smem_raw;
-    T *four = (T *)A.four + J.four_off + (long)g * (C.T + 1) * C.nhp * 32;
+    T *four = (T *)((char *)four_base + J.four_off) + (long)g * (C.T + 1) * C.nhp * 32;
     if (fft_fixed(C.nlon)) {
         AER_FFT_DISPATCH(fft_anal, J, C, g, p0, smem, four)
     } else if constexpr (GENERIC) {
         fft_anal_generic<T>(J, C, g, sub, p0, smem, four);
+    }
+}
+
+template <int PM, bool GENERIC, bool PLAN>
+__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_synth(const LaunchArgs A)
+{
+    extern __shared__ char smem_raw[];
+    const int task = blockIdx.x;
+    if (task >= A.ntask) return;
+    int g, sub, p0;
+    const Job &J = fft_task<PLAN>(A, task, g, sub, p0);
+    if constexpr (PM == PM_MIXED) {
+        if (J.prec) fft_synth_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        else fft_synth_body<0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+    } else {
+        fft_synth_body<PM, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+    }
+}
+
+template <int PM, bool GENERIC, bool PLAN>
+__global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_anal(const LaunchArgs A)
+{
+    extern __shared__ char smem_raw[];
+    const int task = blockIdx.x;
+    if (task >= A.ntask) return;
+    int g, sub, p0;
+    const Job &J = fft_task<PLAN>(A, task, g, sub, p0);
+    if constexpr (PM == PM_MIXED) {
+        if (J.prec) fft_anal_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        else fft_anal_body<0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+    } else {
+        fft_anal_body<PM, GENERIC>(J, g, sub, p0, smem_raw, A.four);
     }
 }
 

@@ -197,7 +197,15 @@ typedef struct aer_job {
                                      analysis of in2, which is added into out0 */
     double scale0, scale1;        /* grid-side factor of component 0 / 1 (0 means 1) */
     int accumulate;               /* analysis: add into out0/out1 instead of overwriting */
+    int prec;                     /* AER_JOB_PREC_*: compute type of this job */
 } aer_job;
+
+/* Per-job compute type. Jobs of different types may share one launch (the kernels
+ * branch per job), e.g. an FP32 transform set whose cancellation-prone analysis
+ * (D_t = div(A) - lap(Phi+K)) runs in FP64. */
+#define AER_JOB_PREC_HANDLE 0     /* the handle's aer_sht_set_precision setting */
+#define AER_JOB_PREC_FP64   1
+#define AER_JOB_PREC_FP32   2
 
 /* All jobs must be of one direction (all AER_JOB_SYNTH_* or all AER_JOB_ANAL_*);
  * at most AER_MAX_JOBS jobs per call. Asynchronous on the handle's stream. */

@@ -20,10 +20,10 @@ def torch():
     return t
 
 
-def job(L, kind, n, ins, outs, modes=(0, 0, 0), sc=(0.0, 0.0), acc=0, strides=(0, 0)):
+def job(L, kind, n, ins, outs, modes=(0, 0, 0), sc=(0.0, 0.0), acc=0, strides=(0, 0), prec=0):
     p = [t.data_ptr() for t in ins] + [0] * (3 - len(ins))
     q = [t.data_ptr() for t in outs] + [0] * (2 - len(outs))
-    return L.AerJob(kind, n, *p, *q, strides[0], strides[1], *modes, sc[0], sc[1], acc)
+    return L.AerJob(kind, n, *p, *q, strides[0], strides[1], *modes, sc[0], sc[1], acc, prec)
 
 
 def run(L, sht, jobs):
@@ -88,6 +88,70 @@ def test_job_validation(torch):
         run(L, s, [job(L, L.AER_JOB_SYNTH_SCALAR, 1, [x], [g], strides=(10, 0))])
     with pytest.raises(L.AerError):
         run(L, s, [job(L, 9, 1, [x], [g])])
+    with pytest.raises(L.AerError):
+        run(L, s, [job(L, L.AER_JOB_SYNTH_SCALAR, 1, [x], [g], prec=7)])
+
+
+def _mixed_case(torch, L, s, T, nlev):
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    vor = dev(red_spectra(T, nlev, 51, zero_l0=True) * 1e-5)
+    div = dev(red_spectra(T, nlev, 52, zero_l0=True) * 1e-6)
+    tem = dev(red_spectra(T, nlev, 53))
+    G = lambda: torch.empty((nlev, s.nlat, s.nlon), dtype=torch.float64, device="cuda")
+    S = lambda: torch.empty((nlev, s.nlm), dtype=torch.complex128, device="cuda")
+    bufs = {"u": G(), "v": G(), "tg": G(), "gx": G(), "gy": G(), "tv": S(), "td": S(), "tt": S()}
+
+    def syn(precs):
+        return [job(L, L.AER_JOB_SYNTH_VECTOR, nlev, [div, vor], [bufs["v"], bufs["u"]],
+                    (L.AER_MODE_S_FROM_DIV, L.AER_MODE_T_FROM_VOR, 0), (-1.0, 1.0), prec=precs[0]),
+                job(L, L.AER_JOB_SYNTH_SCALAR, nlev, [tem], [bufs["tg"]], prec=precs[1]),
+                job(L, L.AER_JOB_SYNTH_GRAD, nlev, [tem], [bufs["gy"], bufs["gx"]], sc=(-1 / A, 1 / A), prec=precs[2])]
+
+    def ana(precs):
+        ek = (bufs["tg"] * 10.0).contiguous()
+        bufs["ek"] = ek
+        return [job(L, L.AER_JOB_ANAL_VECTOR, nlev, [bufs["v"], bufs["u"], ek], [bufs["td"], bufs["tv"]],
+                    (L.AER_MODE_DIV_FROM_S, L.AER_MODE_VOR_FROM_T, L.AER_MODE_NEG_LAP), (-1.0, 1.0), prec=precs[0]),
+                job(L, L.AER_JOB_ANAL_SCALAR, nlev, [bufs["gx"]], [bufs["tt"]], prec=precs[1])]
+    return bufs, syn, ana
+
+
+@pytest.mark.parametrize("T", [31, 42])
+def test_mixed_precision_launch_is_bit_identical(torch, T):
+    """One launch with FP64 and FP32 jobs gives, bit for bit, what the same jobs give
+    in single-precision launches (each job runs its own precision's code path)."""
+    from aeros_amd import _lib as L
+    from aeros_amd.spectral import AerosSht
+    F64, F32 = L.AER_JOB_PREC_FP64, L.AER_JOB_PREC_FP32
+    s = AerosSht(T, precision="fp32")
+    nlev = 6
+    bufs, syn, ana = _mixed_case(torch, L, s, T, nlev)
+    outs = {}
+    for tag, ps, pa in [("mixed", (F32, F64, F32), (F64, F32)),
+                        ("f32", (F32, F32, F32), (F32, F32)),
+                        ("f64", (F64, F64, F64), (F64, F64))]:
+        run(L, s, syn(ps))
+        torch.cuda.synchronize()
+        got = {k: bufs[k].clone() for k in ("u", "v", "tg", "gx", "gy")}
+        # the analysis always reads the FP64-synthesized grids so that only its own precision differs
+        if tag == "mixed":
+            run(L, s, syn((F64, F64, F64)))
+        run(L, s, ana(pa))
+        torch.cuda.synchronize()
+        got.update({k: bufs[k].clone() for k in ("tv", "td", "tt")})
+        outs[tag] = got
+    m, f32, f64 = outs["mixed"], outs["f32"], outs["f64"]
+    for k in ("u", "v", "gx", "gy"):
+        assert torch.equal(m[k], f32[k]), k
+    assert torch.equal(m["tg"], f64["tg"])
+    assert torch.equal(m["tv"], f64["tv"]) and torch.equal(m["td"], f64["td"])
+    # the FP32 analysis in the mixed launch read FP64-synthesized grids, as the f64 run did;
+    # compare it with an FP32 launch on those same grids
+    run(L, s, syn((F64, F64, F64)))
+    run(L, s, [ana((F64, F32))[1]])
+    torch.cuda.synchronize()
+    assert torch.equal(m["tt"], bufs["tt"])
+    assert rel(m["tt"].cpu().numpy(), f64["tt"].cpu().numpy()) < 1e-5
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -19,9 +19,9 @@ def torch():
     return t
 
 
-def make_member(torch, L, T, nlev, cfgid):
+def make_member(torch, L, T, nlev, cfgid, precision="fp64"):
     from aeros_amd.spectral import AerosSht
-    s = AerosSht(T)
+    s = AerosSht(T, precision=precision)
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     m = {"sht": s, "T": T, "nlev": nlev,
          "q": dev(red_spectra(T, nlev, cfgid)),
@@ -114,6 +114,32 @@ def test_mixed_truncation_plan(torch):
         L.lib().aer_plan_destroy(plan)
 
 
+def test_plan_mixes_precisions_bit_identically(torch):
+    """Handles of both precisions in one plan: every member's outputs equal, bit for
+    bit, its own handle's fused launches."""
+    from aeros_amd import _lib as L
+    members = [make_member(torch, L, T, n, 80 + 3 * i, prec)
+               for i, (T, n, prec) in enumerate([(21, 4, "fp32"), (42, 6, "fp64"), (31, 5, "fp32"), (42, 3, "fp32")])]
+    plan = build_plan(L, members)
+    try:
+        L.check(L.lib().aer_plan_run(plan, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        torch.cuda.synchronize()
+        got = outputs(members)
+        for m in members:
+            for key in ("g", "u", "v", "qa", "va", "da"):
+                m[key].zero_()
+            s = m["sht"]
+            s._bind_stream()
+            L.check(L.lib().aer_run_jobs_d(s._h, 2, (L.AerJob * 2)(*m["jobs"][:2])))
+            L.check(L.lib().aer_run_jobs_d(s._h, 2, (L.AerJob * 2)(*m["jobs"][2:])))
+        torch.cuda.synchronize()
+        for m, o in zip(members, got):
+            for key in o:
+                assert torch.equal(m[key], o[key]), (m["T"], key)
+    finally:
+        L.lib().aer_plan_destroy(plan)
+
+
 def test_plan_validation(torch):
     from aeros_amd import _lib as L
     from aeros_amd.spectral import AerosSht
@@ -121,9 +147,10 @@ def test_plan_validation(torch):
     x = torch.zeros((1, a.nlm), dtype=torch.complex128, device="cuda")
     g = torch.zeros((1, a.nlat, a.nlon), dtype=torch.float64, device="cuda")
     j = L.AerJob(L.AER_JOB_SYNTH_SCALAR, 1, x.data_ptr(), 0, 0, g.data_ptr(), 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0)
+    bad = L.AerJob(L.AER_JOB_SYNTH_SCALAR, 1, x.data_ptr(), 0, 0, g.data_ptr(), 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0, 9)
     plan = ctypes.c_void_p()
     cf = (ctypes.c_void_p * 2)(a._h, b._h)
-    with pytest.raises(L.AerError):   # precisions differ
-        L.check(L.lib().aer_plan_create(2, cf, (L.AerJob * 2)(j, j), ctypes.byref(plan)))
+    with pytest.raises(L.AerError):   # unknown job precision
+        L.check(L.lib().aer_plan_create(2, cf, (L.AerJob * 2)(j, bad), ctypes.byref(plan)))
     with pytest.raises(L.AerError):   # empty
         L.check(L.lib().aer_plan_create(0, cf, (L.AerJob * 2)(j, j), ctypes.byref(plan)))

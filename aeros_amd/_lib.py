@@ -86,6 +86,9 @@ SIGNATURES["aer_run_jobs_d"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob)])
 SIGNATURES["aer_plan_create"] = (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(AerJob), ctypes.POINTER(_vp)])
 SIGNATURES["aer_plan_run"] = (_c_int, [_vp, _vp])
 SIGNATURES["aer_plan_destroy"] = (_c_int, [_vp])
+SIGNATURES["aer_vgrid_create"] = (_c_int, [_c_int, _dp, _dp, ctypes.POINTER(_vp)])
+SIGNATURES["aer_vgrid_destroy"] = (_c_int, [_vp])
+SIGNATURES["aer_column_terms_d"] = (_c_int, [_vp, _vp, _c_int] + [_vp] * 16)
 SIGNATURES["aer_dev_malloc"] = (_c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)])
 SIGNATURES["aer_dev_free"] = (_c_int, [_vp])
 SIGNATURES["aer_memcpy_h2d"] = (_c_int, [_vp, _vp, ctypes.c_size_t])

@@ -1,0 +1,67 @@
+"""Python mirror of the grid-space column work of an aeros dynamics step over
+the C ABI: the hybrid vertical coordinate (reference aeros_vgrid_class,
+src/dynamics/aeros_vertical.f90:46-55), the Simmons-Burridge column terms
+(src/dynamics/aeros_tendency.f90:475-630) and the Held-Suarez forcing
+(src/physics/aeros_held_suarez.f90:177-223), run on the GPU in one kernel.
+
+Arrays are torch CUDA tensors in the layouts of the transform halves: 3-D grid
+fields (nlev, nlat, nlon), 2-D (nlat, nlon), float64.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+
+class VGrid:
+    """Device copy of the hybrid coefficients A (Pa), B (-) at nlev+1 half levels, top first."""
+
+    def __init__(self, a_half, b_half):
+        a = np.ascontiguousarray(a_half, dtype=np.float64)
+        b = np.ascontiguousarray(b_half, dtype=np.float64)
+        if a.shape != b.shape or a.ndim != 1:
+            raise ValueError("a_half and b_half must be 1-D of equal length nlev+1")
+        self.nlev = a.size - 1
+        self.a_half, self.b_half = a.copy(), b.copy()
+        h = ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().aer_vgrid_create(self.nlev, a.ctypes.data_as(dp), b.ctypes.data_as(dp), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().aer_vgrid_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def column_terms(sht, vg, grid, phis=None, held_suarez=False, out=None):
+    """Grid right-hand sides of every column from the dict that
+    AerosSht.tendency_grid_transforms returns. Returns ae, an, ek, dtdt (3-D) and
+    dlnpsdt (2-D) as a dict; inputs are left unchanged."""
+    import torch
+    u = grid["u"]
+    nlev = u.shape[0]
+    if nlev != vg.nlev or u.shape[1:] != (sht.nlat, sht.nlon):
+        raise ValueError("grid fields do not match the vertical grid / transform grid")
+    for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy", "lnps_g", "dlnpsdx", "dlnpsdy"):
+        t = grid[k]
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+    if phis is not None and not (phis.is_cuda and phis.dtype == torch.float64 and phis.is_contiguous()):
+        raise ValueError("phis: expected a contiguous float64 CUDA tensor")
+    if out is None:
+        out = {k: torch.empty_like(u) for k in ("ae", "an", "ek", "dtdt")}
+        out["dlnpsdt"] = torch.empty_like(grid["lnps_g"])
+    sht._bind_stream()
+    check(lib().aer_column_terms_d(
+        sht._h, vg._h, 1 if held_suarez else 0,
+        *[grid[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
+                                       "lnps_g", "dlnpsdx", "dlnpsdy")],
+        phis.data_ptr() if phis is not None else None,
+        *[out[k].data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")]))
+    return out

@@ -3,6 +3,7 @@
 // FFT plans, and the job builder that turns each API call into one launch per
 // stage (see sht_device.h for the kernels).
 #include "sht_device.h"
+#include "column_device.h"
 #include "../../include/aer_sht.h"
 
 #include <algorithm>
@@ -68,6 +69,7 @@ struct aer_sht {
     std::vector<int> radix, twoff;
     // device data
     DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
+    DevBuf sinlat;                          // cos(colatitude) per row, for the column kernel
     std::vector<double> tw_host, twN_host;  // per-pass twiddles; e^{-2 pi i k/nlon}
     // per-kernel event timing
     bool timing = false;
@@ -609,6 +611,7 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
         if (nlat - 1 - j == j) wp[j] *= 0.5;
     }
     if ((rc = upload(h->wpair, wp.data(), wp.size() * sizeof(double)))) return rc;
+    if ((rc = upload(h->sinlat, h->ct.data(), h->ct.size() * sizeof(double)))) return rc;
     // FFT plan: radices 4, 2, 3, 5 and the forward twiddles of each pass
     h->radix.clear();
     h->twoff.clear();
@@ -752,6 +755,7 @@ int aer_sht_destroy(aer_sht_t h)
     h->four.release();
     h->stage_s.release();
     h->stage_g.release();
+    h->sinlat.release();
     delete h;
     return AER_OK;
 }
@@ -1196,6 +1200,81 @@ int aer_SHsph_to_spat(aer_sht_t h, const double *slm, double *vt, double *vp)
     HIP_TRY(hipMemcpyAsync(vt, dg, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipMemcpyAsync(vp, dg + gs, gs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return AER_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ column work
+
+struct aer_vgrid {
+    int nlev = 0;
+    bool ztop = false;
+    DevBuf ab;   // A then B, nlev+1 each
+};
+
+extern "C" {
+
+int aer_vgrid_create(int nlev, const double *a_half, const double *b_half, aer_vgrid_t *vg)
+{
+    if (!vg) return fail(AER_EINVAL, "aer_vgrid_create: null output");
+    *vg = nullptr;
+    if (nlev < 2 || nlev > AER_COL_MAXLEV || !a_half || !b_half)
+        return fail(AER_EINVAL, "aer_vgrid_create: need 2 <= nlev <= 64 and both coefficient arrays");
+    for (int j = 0; j < nlev; ++j)
+        if (!(a_half[j + 1] + b_half[j + 1] * 1.0e5 > a_half[j] + b_half[j] * 1.0e5))
+            return fail(AER_EINVAL, "aer_vgrid_create: half-level pressures must increase downward");
+    std::vector<double> ab(2 * (nlev + 1));
+    std::copy(a_half, a_half + nlev + 1, ab.begin());
+    std::copy(b_half, b_half + nlev + 1, ab.begin() + nlev + 1);
+    aer_vgrid *g = new aer_vgrid();
+    g->nlev = nlev;
+    g->ztop = a_half[0] <= 0.0 && b_half[0] <= 0.0;   // src/dynamics/aeros_tendency.f90:499
+    int rc = upload(g->ab, ab.data(), ab.size() * sizeof(double));
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *vg = g;
+    return AER_OK;
+}
+
+int aer_vgrid_destroy(aer_vgrid_t vg)
+{
+    if (!vg) return AER_OK;
+    vg->ab.release();
+    delete vg;
+    return AER_OK;
+}
+
+int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
+                       const double *u, const double *v, const double *vor_g, const double *div_g,
+                       const double *t_g, const double *dtdx, const double *dtdy,
+                       const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                       const double *phis,
+                       double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
+{
+    if (!h || !vg) return fail(AER_EINVAL, "aer_column_terms_d: null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "aer_column_terms_d: handle without a grid");
+    if (!u || !v || !vor_g || !div_g || !t_g || !dtdx || !dtdy || !lnps_g || !dlnpsdx || !dlnpsdy ||
+        !ae || !an || !ek || !dtdt || !dlnpsdt)
+        return fail(AER_EINVAL, "aer_column_terms_d: missing array");
+    aer::ColumnArgs C{};
+    C.nlon = h->nlon;
+    C.ncol = h->nlon * h->nlat;
+    C.nlev = vg->nlev;
+    C.hs = held_suarez ? 1 : 0;
+    C.ztop = vg->ztop ? 1 : 0;
+    C.A = (const double *)vg->ab.p;
+    C.B = C.A + vg->nlev + 1;
+    C.sinlat = (const double *)h->sinlat.p;
+    C.u = u; C.v = v; C.vor = vor_g; C.div = div_g; C.t = t_g; C.dtdx = dtdx; C.dtdy = dtdy;
+    C.lnps = lnps_g; C.dlnpsdx = dlnpsdx; C.dlnpsdy = dlnpsdy; C.phis = phis;
+    C.ae = ae; C.an = an; C.ek = ek; C.dtdt = dtdt; C.dlnpsdt = dlnpsdt;
+    const int nb = (C.ncol + AER_COL_CB - 1) / AER_COL_CB;
+    const size_t lds = (size_t)(2 * (C.nlev + 1) + (7 * C.nlev + 1) * AER_COL_CB) * sizeof(double);
+    hipLaunchKernelGGL(aer::k_column_terms, dim3(nb), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
+    HIP_TRY(hipGetLastError());
     return AER_OK;
 }
 

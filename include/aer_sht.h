@@ -226,6 +226,29 @@ int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_pl
 int aer_plan_run(aer_plan_t plan, void *stream);
 int aer_plan_destroy(aer_plan_t plan);
 
+/* ---- grid-space column work between the transform halves of a step.
+ *
+ * Vertical coordinate: the hybrid coefficients of the reference's
+ * aeros_vgrid_class (src/dynamics/aeros_vertical.f90:46-55), A (Pa) and B (-) at
+ * the nlev+1 half levels, top first, copied to the device once. 2 <= nlev <= 64. */
+typedef struct aer_vgrid *aer_vgrid_t;
+int aer_vgrid_create(int nlev, const double *a_half, const double *b_half, aer_vgrid_t *vg);
+int aer_vgrid_destroy(aer_vgrid_t vg);
+
+/* Grid right-hand sides of every column (column_terms + vert_adv,
+ * src/dynamics/aeros_tendency.f90:475-630): from the synthesized fields of
+ * aer_tendency_grid_transforms_d (3-D arrays [nlev][nlat][nlon], 2-D [nlat][nlon],
+ * device pointers) to ae, an, ek = Phi + K, dtdt and dlnpsdt. phis (surface
+ * geopotential) may be NULL for a flat surface. held_suarez != 0 adds the
+ * Held-Suarez forcing in the same pass (aeros_hs_apply,
+ * src/physics/aeros_held_suarez.f90:177-223). Asynchronous on the handle's stream. */
+int aer_column_terms_d(aer_sht_t cfg, aer_vgrid_t vg, int held_suarez,
+                       const double *u, const double *v, const double *vor_g, const double *div_g,
+                       const double *t_g, const double *dtdx, const double *dtdy,
+                       const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                       const double *phis,
+                       double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt);
+
 /* ---- device memory for hosts without a HIP binding of their own (Fortran): thin
  * wrappers of hipMalloc / hipFree / synchronous hipMemcpy. No reference counterpart. */
 int aer_dev_malloc(size_t bytes, void **ptr);

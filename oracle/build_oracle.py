@@ -1,6 +1,7 @@
 """Build recipe for the FP64 CPU oracle (test infrastructure only).
 
-Compiles oracle/sht_oracle.c with gcc and OpenMP into oracle/liborc.so. No
+Compiles oracle/sht_oracle.c and oracle/column_oracle.c with gcc and OpenMP
+into oracle/liborc.so. No
 reference build exists for this path: the reference's transform arithmetic is
 the third-party SHTns C library, which the reference does not vendor, so there
 is no oracle/_ref target.
@@ -14,9 +15,9 @@ LIB = os.path.join(HERE, "liborc.so")
 
 
 def build(verbose: bool = False) -> str:
-    src = os.path.join(HERE, "sht_oracle.c")
+    srcs = [os.path.join(HERE, f) for f in ("sht_oracle.c", "column_oracle.c")]
     cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c11",
-           "-Wall", "-Wno-unused-function", src, "-o", LIB, "-lm"]
+           "-Wall", "-Wno-unused-function", *srcs, "-o", LIB, "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -59,6 +59,16 @@ def lib():
         L.orc_tendency_spectral.argtypes = [cp, ctypes.c_int] + [_dp] * 9
         L.orc_bench_set.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.orc_bench_set.restype = ctypes.c_double
+        d = ctypes.c_double
+        L.orc_vgrid_init.argtypes = [ctypes.c_int, d, d, d, d, d, _dp, _dp]
+        L.orc_vgrid_init.restype = ctypes.c_int
+        L.orc_vgrid_pressure.argtypes = [ctypes.c_int, _dp, _dp, d, _dp, _dp, _dp]
+        L.orc_vgrid_alpha.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_hydrostatic.argtypes = [ctypes.c_int, d, _dp, _dp, _dp]
+        for name, n in [("orc_hs_teq", 3), ("orc_hs_kt", 2), ("orc_hs_kv", 1)]:
+            getattr(L, name).argtypes = [d] * n
+            getattr(L, name).restype = d
+        L.orc_column_terms.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [_dp] * 11 + [_dp] * 5 + [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -244,3 +254,69 @@ class Oracle:
         so = np.empty_like(spec_in)
         return lib().orc_bench_set(self._c, nlev, nrep, nthreads, spec_in.ctypes.data_as(_dp),
                                    _p(gw), so.ctypes.data_as(_dp))
+
+
+# ------------------------------------------------------------- column physics
+
+R_D, CP_D, OMEGA, P0 = 287.05, 1004.64, 7.292e-5, 1.0e5
+KAPPA = R_D / CP_D
+
+
+def vgrid_init(nlev, stretch_a=0.4, stretch_r=2.0, sigma_t=0.2, p_top=1000.0, ps_ref=P0):
+    """Hybrid coefficients (A, B) at nlev+1 half levels, top first; the reference
+    generator's defaults are its production grid (src/dynamics/aeros_vertical.f90:178-186)."""
+    A, B = np.zeros(nlev + 1), np.zeros(nlev + 1)
+    rc = lib().orc_vgrid_init(nlev, stretch_a, stretch_r, sigma_t, p_top, ps_ref, _p(A), _p(B))
+    if rc != 0:
+        raise ValueError("invalid vertical-grid parameters")
+    return A, B
+
+
+def vgrid_pressure(A, B, ps):
+    n = len(A) - 1
+    ph, pf, dp = np.zeros(n + 1), np.zeros(n), np.zeros(n)
+    lib().orc_vgrid_pressure(n, _p(np.ascontiguousarray(A)), _p(np.ascontiguousarray(B)), ps, _p(ph), _p(pf), _p(dp))
+    return ph, pf, dp
+
+
+def vgrid_alpha(p_half):
+    n = len(p_half) - 1
+    al, dl = np.zeros(n), np.zeros(n)
+    lib().orc_vgrid_alpha(n, _p(np.ascontiguousarray(p_half)), _p(al), _p(dl))
+    return al, dl
+
+
+def hydrostatic(phis, temp, p_half):
+    n = len(temp)
+    phi = np.zeros(n)
+    lib().orc_hydrostatic(n, phis, _p(np.ascontiguousarray(temp, dtype=np.float64)),
+                          _p(np.ascontiguousarray(p_half)), _p(phi))
+    return phi
+
+
+def hs_teq(p, sinlat2, coslat2):
+    return lib().orc_hs_teq(p, sinlat2, coslat2)
+
+
+def hs_kt(sigma, coslat4):
+    return lib().orc_hs_kt(sigma, coslat4)
+
+
+def hs_kv(sigma):
+    return lib().orc_hs_kv(sigma)
+
+
+def column_terms(A, B, sinlat, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy,
+                 phis, hs=False):
+    """Grid right-hand sides (ae, an, ek, dtdt, dlnpsdt) of every column; 3-D inputs
+    are (nlev, nlat, nlon), 2-D (nlat, nlon)."""
+    nlev, nlat, nlon = u.shape
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    ins3 = [c(x) for x in (u, v, vor_g, div_g, t_g, dtdx, dtdy)]
+    ins2 = [c(x) for x in (lnps_g, dlnpsdx, dlnpsdy, phis)]
+    ae, an, ek, dtdt = (np.zeros((nlev, nlat, nlon)) for _ in range(4))
+    dlnpsdt = np.zeros((nlat, nlon))
+    lib().orc_column_terms(nlon, nlat, nlev, _p(c(A)), _p(c(B)), _p(c(sinlat)),
+                           *[_p(x) for x in ins3 + ins2],
+                           _p(ae), _p(an), _p(ek), _p(dtdt), _p(dlnpsdt), 1 if hs else 0)
+    return ae, an, ek, dtdt, dlnpsdt

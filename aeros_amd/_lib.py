@@ -89,6 +89,16 @@ SIGNATURES["aer_plan_destroy"] = (_c_int, [_vp])
 SIGNATURES["aer_vgrid_create"] = (_c_int, [_c_int, _dp, _dp, ctypes.POINTER(_vp)])
 SIGNATURES["aer_vgrid_destroy"] = (_c_int, [_vp])
 SIGNATURES["aer_column_terms_d"] = (_c_int, [_vp, _vp, _c_int] + [_vp] * 16)
+SIGNATURES["aer_semiimp_create"] = (_c_int, [_vp, _vp, _c_double, _c_double, _c_double, _c_double, ctypes.POINTER(_vp)])
+SIGNATURES["aer_semiimp_set_step"] = (_c_int, [_vp, _c_double])
+SIGNATURES["aer_semiimp_set_diffusion"] = (_c_int, [_vp, _c_double, _c_double, _c_int])
+SIGNATURES["aer_semiimp_gwspeed"] = (_c_int, [_vp, _dp])
+SIGNATURES["aer_semiimp_destroy"] = (_c_int, [_vp])
+SIGNATURES["aer_semiimp_linear_d"] = (_c_int, [_vp] + [_vp] * 6)
+SIGNATURES["aer_semiimp_step_d"] = (_c_int, [_vp] + [_vp] * 12)
+SIGNATURES["aer_leapfrog_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_vp] * 16)
+SIGNATURES["aer_mass_d"] = (_c_int, [_vp, _vp, _c_int, _c_double, _vp, _vp])
+AER_STEP_SEMI_IMPLICIT, AER_STEP_DIFFUSE, AER_STEP_FILTER = 1, 2, 4
 SIGNATURES["aer_dev_malloc"] = (_c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)])
 SIGNATURES["aer_dev_free"] = (_c_int, [_vp])
 SIGNATURES["aer_memcpy_h2d"] = (_c_int, [_vp, _vp, ctypes.c_size_t])

@@ -4,6 +4,7 @@
 // stage (see sht_device.h for the kernels).
 #include "sht_device.h"
 #include "column_device.h"
+#include "step_device.h"
 #include "../../include/aer_sht.h"
 
 #include <algorithm>
@@ -70,6 +71,8 @@ struct aer_sht {
     // device data
     DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
     DevBuf sinlat;                          // cos(colatitude) per row, for the column kernel
+    DevBuf wfull, massg;                    // Gauss weights (nlat) and a grid for the mass sums
+    double c00 = 0.0;                       // (0,0) coefficient of the constant field 1
     std::vector<double> tw_host, twN_host;  // per-pass twiddles; e^{-2 pi i k/nlon}
     // per-kernel event timing
     bool timing = false;
@@ -472,6 +475,30 @@ int run_jobs(aer_sht *h, JobSpec *specs, int njob, bool synth)
     return launch_table(h, h->stream, L, B, synth);
 }
 
+// The (0,0) coefficient of the constant field 1, read back by an FP64 analysis
+// rather than written down (reference src/dynamics/aeros_timestep.f90:573-588).
+int read_c00(aer_sht *h)
+{
+    const size_t ng = (size_t)h->nlon * h->nlat;
+    std::vector<double> one(ng, 1.0);
+    int rc = upload(h->massg, one.data(), ng * sizeof(double));
+    if (rc) return rc;
+    if ((rc = h->stage_s.ensure(2 * sizeof(double) * h->nlm))) return rc;
+    JobSpec s{aer::ANA_SCALAR, 1};
+    s.gin0 = (const double *)h->massg.p;
+    s.gstride = (long)ng;
+    s.sout0 = (double *)h->stage_s.p;
+    s.sstride = 2L * h->nlm;
+    s.prec = 0;
+    if ((rc = run_jobs(h, &s, 1, false))) return rc;
+    double c[2];
+    HIP_TRY(hipMemcpyAsync(c, h->stage_s.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->c00 = c[0];
+    if (std::fabs(h->c00) < 1e-12) return fail(AER_EHIP, "aer_sht_set_grid: the (0,0) coefficient of 1 read back as ~0");
+    return AER_OK;
+}
+
 long dense_s(const aer_sht *h) { return 2L * h->nlm; }
 long dense_g(const aer_sht *h) { return (long)h->nlon * h->nlat; }
 
@@ -612,6 +639,8 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     }
     if ((rc = upload(h->wpair, wp.data(), wp.size() * sizeof(double)))) return rc;
     if ((rc = upload(h->sinlat, h->ct.data(), h->ct.size() * sizeof(double)))) return rc;
+    if ((rc = upload(h->wfull, h->w.data(), h->w.size() * sizeof(double)))) return rc;
+    if ((rc = h->massg.ensure((size_t)nlat * nphi * sizeof(double)))) return rc;
     // FFT plan: radices 4, 2, 3, 5 and the forward twiddles of each pass
     h->radix.clear();
     h->twoff.clear();
@@ -649,7 +678,7 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     h->tw32.release();
     h->twN32.release();
     h->grid_set = true;
-    return AER_OK;
+    return read_c00(h);
 }
 
 int aer_sht_host_tables(int lmax, int nlat, double *ct, double *w, double *P, double *D, double *Q)
@@ -756,6 +785,8 @@ int aer_sht_destroy(aer_sht_t h)
     h->stage_s.release();
     h->stage_g.release();
     h->sinlat.release();
+    h->wfull.release();
+    h->massg.release();
     delete h;
     return AER_OK;
 }
@@ -1210,6 +1241,7 @@ int aer_SHsph_to_spat(aer_sht_t h, const double *slm, double *vt, double *vp)
 struct aer_vgrid {
     int nlev = 0;
     bool ztop = false;
+    std::vector<double> A, B;   // host copies
     DevBuf ab;   // A then B, nlev+1 each
 };
 
@@ -1230,6 +1262,8 @@ int aer_vgrid_create(int nlev, const double *a_half, const double *b_half, aer_v
     aer_vgrid *g = new aer_vgrid();
     g->nlev = nlev;
     g->ztop = a_half[0] <= 0.0 && b_half[0] <= 0.0;   // src/dynamics/aeros_tendency.f90:499
+    g->A.assign(a_half, a_half + nlev + 1);
+    g->B.assign(b_half, b_half + nlev + 1);
     int rc = upload(g->ab, ab.data(), ab.size() * sizeof(double));
     if (rc) {
         delete g;
@@ -1274,6 +1308,337 @@ int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
     const int nb = (C.ncol + AER_COL_CB - 1) / AER_COL_CB;
     const size_t lds = (size_t)(2 * (C.nlev + 1) + (7 * C.nlev + 1) * AER_COL_CB) * sizeof(double);
     hipLaunchKernelGGL(aer::k_column_terms, dim3(nb), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
+    HIP_TRY(hipGetLastError());
+    return AER_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ step completion
+
+struct aer_semiimp {
+    aer_sht *h = nullptr;
+    int nlev = 0, lmax = 0, nblk = 0;
+    double alpha = 0.5, dt = 0.0;
+    std::vector<double> G, tau, W, nu, pcoef;        // host copies, row-major
+    bool diff = false;
+    double tau_diff = 0.0, tau_div = 0.0;
+    int ndiff = 0;
+    DevBuf dG, dtau, dW, dnu, dpcoef, dMinv, dfvt, dfdiv, dtl, dtc;
+};
+
+namespace {
+
+// Doolittle LU with partial pivoting in place, row-major (the factorization of
+// reference src/dynamics/aeros_semiimp.f90:658-713).
+bool lu_factor(std::vector<double> &a, std::vector<int> &piv, int n)
+{
+    for (int k = 0; k < n; ++k) {
+        int ip = k;
+        double amax = std::fabs(a[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(a[i * n + k]) > amax) {
+                amax = std::fabs(a[i * n + k]);
+                ip = i;
+            }
+        if (amax <= 0.0) return false;
+        piv[k] = ip;
+        if (ip != k)
+            for (int j = 0; j < n; ++j) std::swap(a[k * n + j], a[ip * n + j]);
+        for (int i = k + 1; i < n; ++i) {
+            a[i * n + k] = a[i * n + k] / a[k * n + k];
+            for (int j = k + 1; j < n; ++j) a[i * n + j] = a[i * n + j] - a[i * n + k] * a[k * n + j];
+        }
+    }
+    return true;
+}
+
+void lu_solve(const std::vector<double> &a, const std::vector<int> &piv, double *b, int n)
+{
+    for (int k = 0; k < n; ++k) {
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+        for (int j = k + 1; j < n; ++j) b[j] = b[j] - a[j * n + k] * b[k];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        for (int j = k + 1; j < n; ++j) b[k] = b[k] - a[k * n + j] * b[j];
+        b[k] = b[k] / a[k * n + k];
+    }
+}
+
+// Factors of the degree-l Helmholtz matrices for the current step, inverted column
+// by column through the LU factors, and the diffusion factors for that step.
+int si_build_step(aer_semiimp *s)
+{
+    const int n = s->nlev;
+    const double hh = s->alpha * s->alpha * s->dt * s->dt, a2 = kREarth * kREarth;
+    std::vector<double> minv((size_t)(s->lmax + 1) * n * n), a((size_t)n * n), col(n);
+    std::vector<int> piv(n);
+    for (int l = 0; l <= s->lmax; ++l) {
+        const double lam = (double)(l * (l + 1)) / a2;
+        for (int j = 0; j < n; ++j) {
+            for (int k = 0; k < n; ++k) a[k * n + j] = hh * lam * s->W[k * n + j];
+            a[j * n + j] = a[j * n + j] + 1.0;
+        }
+        if (!lu_factor(a, piv, n)) return fail(AER_EINVAL, "aer_semiimp: singular Helmholtz matrix");
+        double *mi = minv.data() + (size_t)l * n * n;
+        for (int j = 0; j < n; ++j) {
+            std::fill(col.begin(), col.end(), 0.0);
+            col[j] = 1.0;
+            lu_solve(a, piv, col.data(), n);
+            for (int k = 0; k < n; ++k) mi[k * n + j] = col[k];
+        }
+    }
+    int rc = upload(s->dMinv, minv.data(), minv.size() * sizeof(double));
+    if (rc || !s->diff) return rc;
+    // implicit del^ndiff factors (reference src/dynamics/aeros_timestep.f90:560-563, 1673-1726)
+    const double lref = (double)(s->lmax * (s->lmax + 1));
+    const double rate = s->dt / s->tau_diff, rate_div = s->dt / (s->tau_div > 0.0 ? s->tau_div : s->tau_diff);
+    std::vector<double> fvt((size_t)n * (s->lmax + 1)), fdiv(fvt.size());
+    for (int l = 0; l <= s->lmax; ++l) {
+        const double x = (double)(l * (l + 1)) / lref;
+        double r = s->ndiff / 2 > 0 ? x : 1.0;
+        for (int i = 1; i < s->ndiff / 2; ++i) r = r * x;
+        for (int k = 0; k < n; ++k) {
+            fvt[(size_t)k * (s->lmax + 1) + l] = 1.0 / (1.0 + 1.0 * rate * r);
+            fdiv[(size_t)k * (s->lmax + 1) + l] = 1.0 / (1.0 + 1.0 * rate_div * r);
+        }
+    }
+    if ((rc = upload(s->dfvt, fvt.data(), fvt.size() * sizeof(double)))) return rc;
+    return upload(s->dfdiv, fdiv.data(), fdiv.size() * sizeof(double));
+}
+
+int launch_step(aer_semiimp *s, int flags, double eps, double raw_alpha,
+                const double *ov, const double *od, const double *ot, const double *op,
+                double *cv, double *cd, double *ct, double *cp,
+                const double *tv, const double *td, const double *tt, const double *tp,
+                double *nv, double *nd, double *nt, double *np)
+{
+    aer::StepArgs A{};
+    A.nlev = s->nlev;
+    A.lmax = s->lmax;
+    A.nlm = s->h->nlm;
+    A.flags = flags;
+    A.h = s->dt;
+    A.a1 = s->alpha;
+    A.astep = s->alpha * s->dt;
+    A.hh_old = s->alpha * (1.0 - s->alpha) * s->dt * s->dt;
+    A.half_nu = 0.5 * eps;
+    A.raw_a = raw_alpha;
+    A.raw_b = 1.0 - raw_alpha;
+    A.G = (const double *)s->dG.p;
+    A.tau = (const double *)s->dtau.p;
+    A.W = (const double *)s->dW.p;
+    A.nu = (const double *)s->dnu.p;
+    A.pcoef = (const double *)s->dpcoef.p;
+    A.Minv = (const double *)s->dMinv.p;
+    A.fvt = (const double *)s->dfvt.p;
+    A.fdiv = (const double *)s->dfdiv.p;
+    A.task_l = (const int *)s->dtl.p;
+    A.task_c0 = (const int *)s->dtc.p;
+    A.ovor = ov; A.odiv = od; A.otemp = ot; A.olnps = op;
+    A.cvor = cv; A.cdiv = cd; A.ctemp = ct; A.clnps = cp;
+    A.tvor = tv; A.tdiv = td; A.ttemp = tt; A.tlnps = tp;
+    A.nvor = nv; A.ndiv = nd; A.ntemp = nt; A.nlnps = np;
+    const size_t lds = (size_t)(11 * s->nlev + 5) * AER_SI_CB * sizeof(double);
+    hipLaunchKernelGGL(aer::k_step, dim3(s->nblk), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
+    HIP_TRY(hipGetLastError());
+    return AER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_semiimp_create(aer_sht_t h, aer_vgrid_t vg, double t_ref, double ps_ref, double dt_step,
+                       double alpha, aer_semiimp_t *out)
+{
+    if (!out) return fail(AER_EINVAL, "aer_semiimp_create: null output");
+    *out = nullptr;
+    if (!h || !vg) return fail(AER_EINVAL, "aer_semiimp_create: null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "aer_semiimp_create: handle without a grid");
+    if (alpha < 0.5 || alpha > 1.0) return fail(AER_EINVAL, "aer_semiimp_create: alpha must be in [0.5, 1]");
+    if (dt_step <= 0.0 || t_ref <= 0.0 || ps_ref <= 0.0) return fail(AER_EINVAL, "aer_semiimp_create: step, t_ref and ps_ref must be > 0");
+    const int n = vg->nlev;
+    aer_semiimp *s = new aer_semiimp();
+    s->h = h;
+    s->nlev = n;
+    s->lmax = h->lmax;
+    s->alpha = alpha;
+    s->dt = dt_step;
+    // reference column at ps_ref (src/dynamics/aeros_semiimp.f90:222-275)
+    std::vector<double> ph(n + 1), dp(n), al(n), dl(n);
+    for (int j = 0; j <= n; ++j) ph[j] = vg->A[j] + vg->B[j] * ps_ref;
+    for (int k = 0; k < n; ++k) {
+        dp[k] = ph[k + 1] - ph[k];
+        if (ph[k] <= 0.0) {
+            al[k] = std::log(2.0);
+            dl[k] = aer::kMV;
+        } else {
+            const double lr = std::log(ph[k + 1] / ph[k]);
+            al[k] = 1.0 - (ph[k] / dp[k]) * lr;
+            dl[k] = lr;
+        }
+    }
+    if (vg->ztop) dl[0] = 0.0;
+    s->G.assign((size_t)n * n, 0.0);
+    s->tau.assign((size_t)n * n, 0.0);
+    s->W.assign((size_t)n * n, 0.0);
+    s->nu.assign(n, 0.0);
+    s->pcoef.assign(n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        s->G[k * n + k] = aer::kRd * al[k];
+        for (int j = k + 1; j < n; ++j) s->G[k * n + j] = aer::kRd * dl[j];
+        s->tau[k * n + k] = aer::kKappa * t_ref * al[k];
+        for (int j = 0; j < k; ++j) s->tau[k * n + j] = aer::kKappa * t_ref * dl[k] * dp[j] / dp[k];
+        s->nu[k] = dp[k] / ps_ref;
+        s->pcoef[k] = aer::kRd * t_ref;
+    }
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) {
+            double w = s->pcoef[k] * s->nu[j];
+            for (int m = 0; m < n; ++m) w = w + s->G[k * n + m] * s->tau[m * n + j];
+            s->W[k * n + j] = w;
+        }
+    std::vector<int> tl, tc;
+    for (int l = 0; l <= s->lmax; ++l)
+        for (int c0 = 0; c0 < 2 * (l + 1); c0 += AER_SI_CB) {
+            tl.push_back(l);
+            tc.push_back(c0);
+        }
+    s->nblk = (int)tl.size();
+    int rc = 0;
+    if (!rc) rc = upload(s->dG, s->G.data(), s->G.size() * sizeof(double));
+    if (!rc) rc = upload(s->dtau, s->tau.data(), s->tau.size() * sizeof(double));
+    if (!rc) rc = upload(s->dW, s->W.data(), s->W.size() * sizeof(double));
+    if (!rc) rc = upload(s->dnu, s->nu.data(), s->nu.size() * sizeof(double));
+    if (!rc) rc = upload(s->dpcoef, s->pcoef.data(), s->pcoef.size() * sizeof(double));
+    if (!rc) rc = upload(s->dtl, tl.data(), tl.size() * sizeof(int));
+    if (!rc) rc = upload(s->dtc, tc.data(), tc.size() * sizeof(int));
+    if (!rc) rc = si_build_step(s);
+    if (rc) {
+        aer_semiimp_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return AER_OK;
+}
+
+int aer_semiimp_set_step(aer_semiimp_t s, double dt_step)
+{
+    if (!s) return fail(AER_EINVAL, "aer_semiimp_set_step: null object");
+    if (dt_step <= 0.0) return fail(AER_EINVAL, "aer_semiimp_set_step: step must be > 0");
+    s->dt = dt_step;
+    return si_build_step(s);
+}
+
+int aer_semiimp_set_diffusion(aer_semiimp_t s, double tau_diff, double tau_diff_div, int ndiff)
+{
+    if (!s) return fail(AER_EINVAL, "aer_semiimp_set_diffusion: null object");
+    if (tau_diff <= 0.0 || ndiff < 2 || ndiff % 2) return fail(AER_EINVAL, "aer_semiimp_set_diffusion: need tau_diff > 0 and an even ndiff >= 2");
+    s->diff = true;
+    s->tau_diff = tau_diff;
+    s->tau_div = tau_diff_div;
+    s->ndiff = ndiff;
+    return si_build_step(s);
+}
+
+int aer_semiimp_gwspeed(aer_semiimp_t s, double *cmax)
+{
+    if (!s || !cmax) return fail(AER_EINVAL, "aer_semiimp_gwspeed: null argument");
+    const int n = s->nlev;
+    std::vector<double> x(n, 1.0 / std::sqrt((double)n)), y(n);
+    double lam = 0.0;
+    for (int it = 0; it < 200; ++it) {
+        for (int k = 0; k < n; ++k) {
+            y[k] = 0.0;
+            for (int j = 0; j < n; ++j) y[k] = y[k] + s->W[k * n + j] * x[j];
+        }
+        double nrm = 0.0;
+        for (int k = 0; k < n; ++k) nrm += y[k] * y[k];
+        nrm = std::sqrt(nrm);
+        if (nrm <= 0.0) break;
+        for (int k = 0; k < n; ++k) x[k] = y[k] / nrm;
+        lam = 0.0;
+        for (int k = 0; k < n; ++k) lam += x[k] * y[k];
+    }
+    *cmax = std::sqrt(std::max(lam, 0.0));
+    return AER_OK;
+}
+
+int aer_semiimp_destroy(aer_semiimp_t s)
+{
+    if (!s) return AER_OK;
+    for (DevBuf *b : {&s->dG, &s->dtau, &s->dW, &s->dnu, &s->dpcoef, &s->dMinv, &s->dfvt, &s->dfdiv, &s->dtl, &s->dtc})
+        b->release();
+    delete s;
+    return AER_OK;
+}
+
+int aer_semiimp_linear_d(aer_semiimp_t s, const double *div, const double *temp, const double *lnps,
+                         double *ldiv, double *ltemp, double *lps)
+{
+    if (!s) return fail(AER_EINVAL, "aer_semiimp_linear_d: null object");
+    if (!div || !temp || !lnps || !ldiv || !ltemp || !lps) return fail(AER_EINVAL, "aer_semiimp_linear_d: missing array");
+    return launch_step(s, aer::ST_LINEAR, 0.0, 1.0, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, (double *)div, (double *)temp, (double *)lnps,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, ldiv, ltemp, lps);
+}
+
+int aer_semiimp_step_d(aer_semiimp_t s,
+                       const double *old_div, const double *old_temp, const double *old_lnps,
+                       const double *now_div, const double *now_temp, const double *now_lnps,
+                       const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                       double *new_div, double *new_temp, double *new_lnps)
+{
+    if (!s) return fail(AER_EINVAL, "aer_semiimp_step_d: null object");
+    if (!old_div || !old_temp || !old_lnps || !now_div || !now_temp || !now_lnps || !tnd_div || !tnd_temp ||
+        !tnd_lnps || !new_div || !new_temp || !new_lnps)
+        return fail(AER_EINVAL, "aer_semiimp_step_d: missing array");
+    return launch_step(s, aer::ST_SEMI, 0.0, 1.0, nullptr, old_div, old_temp, old_lnps,
+                       nullptr, (double *)now_div, (double *)now_temp, (double *)now_lnps,
+                       nullptr, tnd_div, tnd_temp, tnd_lnps, nullptr, new_div, new_temp, new_lnps);
+}
+
+int aer_leapfrog_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alpha,
+                   const double *old_vor, const double *old_div, const double *old_temp, const double *old_lnps,
+                   double *now_vor, double *now_div, double *now_temp, double *now_lnps,
+                   const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                   double *new_vor, double *new_div, double *new_temp, double *new_lnps)
+{
+    if (!s) return fail(AER_EINVAL, "aer_leapfrog_d: null object");
+    if (flags & ~(AER_STEP_SEMI_IMPLICIT | AER_STEP_DIFFUSE | AER_STEP_FILTER))
+        return fail(AER_EINVAL, "aer_leapfrog_d: unknown flag");
+    if ((flags & AER_STEP_DIFFUSE) && !s->diff)
+        return fail(AER_ESTATE, "aer_leapfrog_d: diffusion requested before aer_semiimp_set_diffusion");
+    if (!old_vor || !old_div || !old_temp || !old_lnps || !now_vor || !now_div || !now_temp || !now_lnps ||
+        !tnd_vor || !tnd_div || !tnd_temp || !tnd_lnps || !new_vor || !new_div || !new_temp || !new_lnps)
+        return fail(AER_EINVAL, "aer_leapfrog_d: missing array");
+    int f = aer::ST_VOR;
+    if (flags & AER_STEP_SEMI_IMPLICIT) f |= aer::ST_SEMI;
+    if (flags & AER_STEP_DIFFUSE) f |= aer::ST_DIFF;
+    if ((flags & AER_STEP_FILTER) && eps_filter > 0.0) f |= aer::ST_RAW;
+    return launch_step(s, f, eps_filter, raw_alpha, old_vor, old_div, old_temp, old_lnps,
+                       now_vor, now_div, now_temp, now_lnps, tnd_vor, tnd_div, tnd_temp, tnd_lnps,
+                       new_vor, new_div, new_temp, new_lnps);
+}
+
+int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out)
+{
+    if (!h) return fail(AER_EINVAL, "aer_mass_d: null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, "aer_mass_d: handle without a grid");
+    if (!lnps) return fail(AER_EINVAL, "aer_mass_d: missing array");
+    if (fix && mass_target <= 0.0) return fail(AER_EINVAL, "aer_mass_d: mass target must be > 0");
+    JobSpec sp{aer::SYN_SCALAR, 1};
+    sp.sin0 = lnps;
+    sp.sstride = 2L * h->nlm;
+    sp.gout0 = (double *)h->massg.p;
+    sp.gstride = (long)h->nlon * h->nlat;
+    sp.prec = 0;   // global mass sums stay FP64
+    int rc = run_jobs(h, &sp, 1, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(aer::k_mass, dim3(1), dim3(256), (size_t)h->nlat * sizeof(double), h->stream,
+                       (const double *)h->massg.p, h->nlon, h->nlat, (const double *)h->wfull.p, 9.80665,
+                       fix ? 1 : 0, mass_target, h->c00, lnps, mass_out, lnr_out);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }

@@ -249,6 +249,62 @@ int aer_column_terms_d(aer_sht_t cfg, aer_vgrid_t vg, int held_suarez,
                        const double *phis,
                        double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt);
 
+/* ---- spectral step completion: the semi-implicit gravity-wave solve
+ * (src/dynamics/aeros_semiimp.f90) and the leapfrog pieces of
+ * aeros_timestep_step (src/dynamics/aeros_timestep.f90:1043-1100, 1225-1276,
+ * 1673-1815). Spectral 3-D fields are complex [nlev][nlm] device arrays, 2-D
+ * (ln p_s) [nlm]; all arithmetic FP64. New-state arrays must not alias the
+ * old, now or tendency arrays. */
+typedef struct aer_semiimp *aer_semiimp_t;
+
+/* aeros_semiimp_init (src/dynamics/aeros_semiimp.f90:171-276): the linear
+ * operators about an isothermal t_ref at ps_ref on the handle's truncation, and
+ * the degree-by-degree Helmholtz factors for a step dt_step (h: 2 dt for a
+ * leapfrog step, dt for the start-up step). alpha in [0.5, 1] is the decentering
+ * weight on X^(n+1). nlev <= 64. */
+int aer_semiimp_create(aer_sht_t cfg, aer_vgrid_t vg, double t_ref, double ps_ref, double dt_step,
+                       double alpha, aer_semiimp_t *si);
+/* aeros_semiimp_set_step (:301-343): refactor for a new step length. */
+int aer_semiimp_set_step(aer_semiimp_t si, double dt_step);
+/* Implicit del^ndiff horizontal diffusion (src/dynamics/aeros_timestep.f90:1673-1726)
+ * with e-folding time tau_diff [s] at l = lmax for vorticity and temperature and
+ * tau_diff_div for divergence (<= 0: tau_diff); rebuilt with every set_step. */
+int aer_semiimp_set_diffusion(aer_semiimp_t si, double tau_diff, double tau_diff_div, int ndiff);
+/* aeros_semiimp_gwspeed (:573-620): fastest linear gravity-wave speed [m/s]. Host only. */
+int aer_semiimp_gwspeed(aer_semiimp_t si, double *cmax);
+int aer_semiimp_destroy(aer_semiimp_t si);
+/* aeros_semiimp_linear (:345-399): L applied to (div, temp, lnps). */
+int aer_semiimp_linear_d(aer_semiimp_t si, const double *div, const double *temp, const double *lnps,
+                         double *ldiv, double *ltemp, double *lps);
+/* aeros_semiimp_step (:462-571): D, T, ln p_s at n+1 from n-1, n and R(X^n). */
+int aer_semiimp_step_d(aer_semiimp_t si,
+                       const double *old_div, const double *old_temp, const double *old_lnps,
+                       const double *now_div, const double *now_temp, const double *now_lnps,
+                       const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                       double *new_div, double *new_temp, double *new_lnps);
+
+/* Sections 2-4 of aeros_timestep_step in one launch: the vorticity leapfrog, the
+ * semi-implicit (AER_STEP_SEMI_IMPLICIT) or explicit update of D, T, ln p_s, the
+ * implicit diffusion (AER_STEP_DIFFUSE; after aer_semiimp_set_diffusion) and the
+ * Robert-Asselin-Williams filter (AER_STEP_FILTER, eps_filter > 0, raw_alpha = 1:
+ * classical Robert-Asselin), which updates the now arrays in place. The step
+ * length is the semi-implicit object's. */
+#define AER_STEP_SEMI_IMPLICIT 1
+#define AER_STEP_DIFFUSE       2
+#define AER_STEP_FILTER        4
+int aer_leapfrog_d(aer_semiimp_t si, int flags, double eps_filter, double raw_alpha,
+                   const double *old_vor, const double *old_div, const double *old_temp, const double *old_lnps,
+                   double *now_vor, double *now_div, double *now_temp, double *now_lnps,
+                   const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                   double *new_vor, double *new_div, double *new_temp, double *new_lnps);
+
+/* Global dry-air mass of a spectral ln p_s (mass_of, src/dynamics/aeros_timestep.f90:1250-1276):
+ * one FP64 synthesis and an FP64 quadrature, written to the device double
+ * *mass_out (may be NULL). With fix != 0 also the mass fixer (mass_fix, :1225-1248):
+ * ln p_s(0,0) += ln(mass_target/mass) c00, ln r written to *lnr_out (device, may be
+ * NULL). c00 is read back from an analysis of the constant 1 at aer_sht_set_grid. */
+int aer_mass_d(aer_sht_t cfg, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out);
+
 /* ---- device memory for hosts without a HIP binding of their own (Fortran): thin
  * wrappers of hipMalloc / hipFree / synchronous hipMemcpy. No reference counterpart. */
 int aer_dev_malloc(size_t bytes, void **ptr);

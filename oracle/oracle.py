@@ -69,6 +69,19 @@ def lib():
             getattr(L, name).argtypes = [d] * n
             getattr(L, name).restype = d
         L.orc_column_terms.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [_dp] * 11 + [_dp] * 5 + [ctypes.c_int]
+        ci = ctypes.c_int
+        L.orc_si_create.restype = ctypes.c_void_p
+        L.orc_si_create.argtypes = [ci, _dp, _dp, d, d, ci, d, d]
+        L.orc_si_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_si_set_step.argtypes = [ctypes.c_void_p, d]
+        L.orc_si_set_step.restype = ci
+        L.orc_si_linear.argtypes = [ctypes.c_void_p, ci, _ip] + [_dp] * 6
+        L.orc_si_step.argtypes = [ctypes.c_void_p, ci, _ip] + [_dp] * 12
+        L.orc_si_gwspeed.argtypes = [ctypes.c_void_p]
+        L.orc_si_gwspeed.restype = d
+        L.orc_diff_factors.argtypes = [ci, d, d, ci, _dp]
+        L.orc_diffuse.argtypes = [ci, ci, ci, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.orc_raw_filter.argtypes = [ctypes.c_long, d, d, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -320,3 +333,100 @@ def column_terms(A, B, sinlat, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnp
                            *[_p(x) for x in ins3 + ins2],
                            _p(ae), _p(an), _p(ek), _p(dtdt), _p(dlnpsdt), 1 if hs else 0)
     return ae, an, ek, dtdt, dlnpsdt
+
+
+# ------------------------------------------------------------- step completion
+
+GRAV = 9.80665   # src/aeros_defs.f90:85
+
+
+def _cv(a):
+    """complex (..., nlm) -> contiguous complex128 viewed as interleaved doubles."""
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _cp(a):
+    return a.ctypes.data_as(_dp)
+
+
+class SemiImplicit:
+    """The reference's aeros_semiimp_class on the CPU (src/dynamics/aeros_semiimp.f90)."""
+
+    def __init__(self, A, B, lmax, dt_step, t_ref=300.0, ps_ref=P0, alpha=0.5):
+        self.nlev, self.lmax = len(A) - 1, lmax
+        self._s = lib().orc_si_create(self.nlev, _p(np.ascontiguousarray(A, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(B, dtype=np.float64)), t_ref, ps_ref,
+                                      lmax, dt_step, alpha)
+        if not self._s:
+            raise ValueError("invalid semi-implicit parameters")
+        self.l_of_lm = np.array([l for m in range(lmax + 1) for l in range(m, lmax + 1)], dtype=np.int32)
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().orc_si_destroy(self._s)
+            self._s = None
+
+    def set_step(self, dt_step):
+        if lib().orc_si_set_step(self._s, dt_step) != 0:
+            raise ValueError("step must be > 0")
+
+    def gwspeed(self):
+        return lib().orc_si_gwspeed(self._s)
+
+    def _li(self):
+        return self.l_of_lm.ctypes.data_as(_ip)
+
+    def linear(self, div, temp, lnps):
+        div, temp, lnps = _cv(div), _cv(temp), _cv(lnps)
+        ld, lt, lp = np.empty_like(div), np.empty_like(temp), np.empty_like(lnps)
+        lib().orc_si_linear(self._s, lnps.size, self._li(), _cp(div), _cp(temp), _cp(lnps),
+                            _cp(ld), _cp(lt), _cp(lp))
+        return ld, lt, lp
+
+    def step(self, old, now, tnd):
+        """old/now/tnd: dicts with div, temp (nlev, nlm) and lnps (nlm,). Returns new (div, temp, lnps)."""
+        ins = [_cv(x[k]) for x in (old, now, tnd) for k in ("div", "temp", "lnps")]
+        nd, nt, nl = np.empty_like(ins[0]), np.empty_like(ins[1]), np.empty_like(ins[2])
+        lib().orc_si_step(self._s, ins[2].size, self._li(), *[_cp(a) for a in ins], _cp(nd), _cp(nt), _cp(nl))
+        return {"div": nd, "temp": nt, "lnps": nl}
+
+
+def diff_factors(lmax, h, tau, ndiff=6):
+    f = np.empty(lmax + 1)
+    lib().orc_diff_factors(lmax, h, tau, ndiff, _p(f))
+    return f
+
+
+def diffuse(l_of_lm, lmax, fac_vt, fac_div, vor, div, temp):
+    """In place on copies; fac_* are (nlev, lmax+1). Returns (vor, div, temp)."""
+    vor, div, temp = _cv(vor).copy(), _cv(div).copy(), _cv(temp).copy()
+    li = np.ascontiguousarray(l_of_lm, dtype=np.int32)
+    lib().orc_diffuse(vor.shape[-1], vor.shape[0], lmax, li.ctypes.data_as(_ip),
+                      _p(np.ascontiguousarray(fac_vt, dtype=np.float64)),
+                      _p(np.ascontiguousarray(fac_div, dtype=np.float64)), _cp(vor), _cp(div), _cp(temp))
+    return vor, div, temp
+
+
+def raw_filter(eps, raw_alpha, old, cur, new):
+    """Returns filtered (cur, new) copies of one field."""
+    old, cur, new = _cv(old), _cv(cur).copy(), _cv(new).copy()
+    lib().orc_raw_filter(2 * old.size, eps, raw_alpha, _cp(old), _cp(cur), _cp(new))
+    return cur, new
+
+
+def mass_of(o, lnps):
+    """Global dry-air mass [kg] of a spectral ln p_s (src/dynamics/aeros_timestep.f90:1250-1276)."""
+    return o.surface_integral(np.exp(o.synth(lnps))) / GRAV
+
+
+def c00(o):
+    """The (0,0) coefficient of the constant field 1, read back by analysis (:573-588)."""
+    return o.anal(np.ones((o.nlat, o.nlon)))[0].real
+
+
+def mass_fix(o, lnps, target):
+    """p_s <- r p_s through the (0,0) coefficient alone (:1225-1248). Returns (lnps, ln r)."""
+    lnps = _cv(lnps).copy()
+    lnr = np.log(target / mass_of(o, lnps))
+    lnps[0] += lnr * c00(o)
+    return lnps, lnr

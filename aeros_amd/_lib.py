@@ -81,7 +81,7 @@ AER_JOB_SYNTH_SCALAR, AER_JOB_SYNTH_VECTOR, AER_JOB_SYNTH_GRAD, AER_JOB_ANAL_SCA
 (AER_MODE_ONE, AER_MODE_S_FROM_DIV, AER_MODE_T_FROM_VOR, AER_MODE_INV_LL1, AER_MODE_DIV_FROM_S,
  AER_MODE_VOR_FROM_T, AER_MODE_NEG_LAP, AER_MODE_ZERO) = range(8)
 AER_MAX_JOBS = 8
-AER_JOB_PREC_HANDLE, AER_JOB_PREC_FP64, AER_JOB_PREC_FP32 = 0, 1, 2
+AER_JOB_PREC_HANDLE, AER_JOB_PREC_FP64, AER_JOB_PREC_FP32, AER_JOB_PREC_SPLIT = 0, 1, 2, 3
 SIGNATURES["aer_run_jobs_d"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob)])
 SIGNATURES["aer_plan_create"] = (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(AerJob), ctypes.POINTER(_vp)])
 SIGNATURES["aer_plan_run"] = (_c_int, [_vp, _vp])

@@ -14,6 +14,32 @@ import numpy as np
 from ._lib import check, lib
 
 
+def vgrid_init(nlev, stretch_a=0.4, stretch_r=2.0, sigma_t=0.2, p_top=1000.0, ps_ref=1.0e5):
+    """Hybrid coefficients (A [Pa], B [-]) at the nlev+1 half levels, top first, from
+    the stretching parameters of aeros_vgrid_init (src/dynamics/aeros_vertical.f90:126-298,
+    the generator branch): level placement sigma = 1 - (a (1-x) + (1-a)(1-x)^r),
+    x = j/nlev, squeezed onto [p_top/ps_ref, 1]; B = 0 above sigma_t, ramping linearly to
+    1 at the surface (pure sigma for sigma_t = 0); A = (sigma - B) ps_ref; endpoints pinned.
+    The defaults are the reference's production grid; Held-Suarez uses stretch_a = 1,
+    sigma_t = 0 (uniform sigma)."""
+    if nlev < 2 or not (0.0 < stretch_a <= 1.0) or not (0.0 <= sigma_t < 1.0):
+        raise ValueError("nlev >= 2, stretch_a in (0, 1] and sigma_t in [0, 1) are required")
+    A, B = np.zeros(nlev + 1), np.zeros(nlev + 1)
+    sig_top = p_top / ps_ref
+    for j in range(nlev + 1):
+        x = j / nlev
+        sh = sig_top + (1.0 - sig_top) * (1.0 - (stretch_a * (1.0 - x) + (1.0 - stretch_a) * (1.0 - x) ** stretch_r))
+        if sigma_t <= 0.0:
+            b = sh
+        elif sh <= sigma_t:
+            b = 0.0
+        else:
+            b = (sh - sigma_t) / (1.0 - sigma_t)
+        A[j], B[j] = (sh - b) * ps_ref, b
+    A[nlev], B[nlev], A[0], B[0] = 0.0, 1.0, p_top, 0.0
+    return A, B
+
+
 class VGrid:
     """Device copy of the hybrid coefficients A (Pa), B (-) at nlev+1 half levels, top first."""
 

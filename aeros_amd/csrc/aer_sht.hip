@@ -70,6 +70,7 @@ struct aer_sht {
     std::vector<int> radix, twoff;
     // device data
     DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
+    DevBuf tabS[3], tabTS[3], expS[3], expTS[3];   // split form: float4 slices and row exponents
     DevBuf sinlat;                          // cos(colatitude) per row, for the column kernel
     DevBuf wfull, massg;                    // Gauss weights (nlat) and a grid for the mass sums
     double c00 = 0.0;                       // (0,0) coefficient of the constant field 1
@@ -231,6 +232,75 @@ int ensure_tables32(aer_sht *h)
     return upload(h->twN32, twN32.data(), twN32.size() * sizeof(float));
 }
 
+// Split-FP32 copies of the tables: each element as five 10-bit integer slices
+// (four as float4, then the fifth in a float array after the float4 region)
+// under a power-of-two scale 2^e shared by the row of A it belongs to --
+// synthesis rows are latitude pairs (over the degrees of one parity of one m),
+// analysis rows are degrees (over all latitude pairs) -- and the row exponents.
+void split_slices(double x, int e, float *s4, float *s5)
+{
+    double y = std::ldexp(x, aer::SPLIT_BITS - e);
+    for (int q = 0; q < 5; ++q) {
+        const double a = std::rint(y);
+        if (q < 4) s4[q] = (float)a;
+        else *s5 = (float)a;
+        y = (y - a) * 1024.0;
+    }
+}
+
+int row_exponent(double amax) { return amax > 0.0 ? std::ilogb(amax) + 1 : 0; }
+
+int ensure_tables_split(aer_sht *h)
+{
+    if (h->tabS[0].p) return AER_OK;
+    const int T = h->lmax, nhp = h->nhp, kp = h->kp, nlm = h->nlm;
+    std::vector<double> P, D, Q;
+    legendre_tables(T, h->nh, nhp, h->ct, P, D, Q);
+    const std::vector<double> *src[3] = {&P, &D, &Q};
+    for (int t = 0; t < 3; ++t) {
+        const std::vector<double> &X = *src[t];
+        const size_t ns = (size_t)nlm * nhp;
+        std::vector<float> sl(ns * 5, 0.f);   // ns float4, then ns fifth slices
+        std::vector<int> ex((size_t)(T + 1) * 2 * nhp, 0);
+        for (int m = 0; m <= T; ++m) {
+            const int lmb = m * (2 * T + 3 - m) / 2;
+            for (int par = 0; par < 2; ++par)
+                for (int j = 0; j < nhp; ++j) {
+                    double amax = 0.0;
+                    for (int l = m + par; l <= T; l += 2) amax = std::max(amax, std::fabs(X[(size_t)(lmb + l - m) * nhp + j]));
+                    const int e = row_exponent(amax);
+                    ex[((size_t)m * 2 + par) * nhp + j] = e;
+                    for (int l = m + par; l <= T; l += 2) {
+                        const size_t o = (size_t)(lmb + l - m) * nhp + j;
+                        split_slices(X[o], e, &sl[o * 4], &sl[ns * 4 + o]);
+                    }
+                }
+        }
+        int rc = upload(h->tabS[t], sl.data(), sl.size() * sizeof(float));
+        if (!rc) rc = upload(h->expS[t], ex.data(), ex.size() * sizeof(int));
+        if (rc) return rc;
+        const std::vector<double> XT = transpose_table(X, T, nhp, kp);
+        const size_t nt = XT.size();
+        std::vector<float> slt(nt * 5, 0.f);
+        std::vector<int> ext((size_t)(T + 1) * 2 * kp, 0);
+        for (int mp = 0; mp < (T + 1) * 2; ++mp)
+            for (int k = 0; k < kp; ++k) {
+                double amax = 0.0;
+                for (int j = 0; j < nhp; ++j) amax = std::max(amax, std::fabs(XT[((size_t)mp * nhp + j) * kp + k]));
+                const int e = row_exponent(amax);
+                ext[(size_t)mp * kp + k] = e;
+                for (int j = 0; j < nhp; ++j) {
+                    const size_t o = ((size_t)mp * nhp + j) * kp + k;
+                    split_slices(XT[o], e, &slt[o * 4], &slt[nt * 4 + o]);
+                }
+            }
+        rc = upload(h->tabTS[t], slt.data(), slt.size() * sizeof(float));
+        if (!rc) rc = upload(h->expTS[t], ext.data(), ext.size() * sizeof(int));
+        if (rc) return rc;
+    }
+    return AER_OK;
+}
+
 aer::CfgDev cfg_dev(const aer_sht *h, int prec)
 {
     aer::CfgDev c{};
@@ -243,11 +313,19 @@ aer::CfgDev cfg_dev(const aer_sht *h, int prec)
     c.nlm = h->nlm;
     c.pb = h->pb;
     c.fb = h->fb;
-    const DevBuf *tab = (prec == AER_PREC_FP32) ? h->tab32 : h->tab64;
+    const DevBuf *tab = (prec == AER_PREC_FP32) ? h->tab32 : (prec == AER_PREC_SPLIT ? h->tabS : h->tab64);
     c.tabP = tab[0].p;
     c.tabD = tab[1].p;
     c.tabQ = tab[2].p;
-    const DevBuf *tabT = (prec == AER_PREC_FP32) ? h->tabT32 : h->tabT64;
+    if (prec == AER_PREC_SPLIT) {
+        c.expP = (const int *)h->expS[0].p;
+        c.expD = (const int *)h->expS[1].p;
+        c.expQ = (const int *)h->expS[2].p;
+        c.expPT = (const int *)h->expTS[0].p;
+        c.expDT = (const int *)h->expTS[1].p;
+        c.expQT = (const int *)h->expTS[2].p;
+    }
+    const DevBuf *tabT = (prec == AER_PREC_FP32) ? h->tabT32 : (prec == AER_PREC_SPLIT ? h->tabTS : h->tabT64);
     c.tabPT = tabT[0].p;
     c.tabDT = tabT[1].p;
     c.tabQT = tabT[2].p;
@@ -328,15 +406,15 @@ struct Table {
 void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
 {
     B = Table{};
-    bool any64 = false, any32 = false;
+    bool any64 = false, any32 = false, anysp = false;
     size_t cxmax = 0;
     for (int k = 0; k < njob; ++k) {
         const JobSpec &s = specs[k];
         if (s.nitem <= 0) continue;
         const aer_sht *h = hs[k];
-        const aer::CfgDev C = cfg_dev(h, s.prec ? AER_PREC_FP32 : AER_PREC_FP64);
-        const size_t cxb = s.prec ? 2 * sizeof(float) : 2 * sizeof(double);
-        (s.prec ? any32 : any64) = true;
+        const aer::CfgDev C = cfg_dev(h, s.prec == 1 ? AER_PREC_FP32 : (s.prec == 2 ? AER_PREC_SPLIT : AER_PREC_FP64));
+        const size_t cxb = s.prec == 1 ? 2 * sizeof(float) : 2 * sizeof(double);
+        (s.prec == 1 ? any32 : (s.prec == 2 ? anysp : any64)) = true;
         cxmax = std::max(cxmax, cxb);
         const int ntpp = ((C.T + 2) / 2 + 15) / 16;
         const int nsub = 8 / h->fb;
@@ -378,7 +456,8 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         B.kmax = std::max(B.kmax, C.T + 1);
         B.jobs.push_back(J);
     }
-    B.pm = any64 && any32 ? aer::PM_MIXED : (any32 ? aer::PM_FP32 : aer::PM_FP64);
+    const int kinds = (int)any64 + (int)any32 + (int)anysp;
+    B.pm = kinds > 1 ? aer::PM_MIXED : (any32 ? aer::PM_FP32 : (anysp ? aer::PM_SPLIT : aer::PM_FP64));
     B.cxmax = cxmax ? cxmax : 16;
 }
 
@@ -432,6 +511,9 @@ int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Ta
         case aer::PM_FP32:
             return plan ? launch_table_pm<aer::PM_FP32, true>(th, st, L, B, synth, tj_leg, tj_fft)
                         : launch_table_pm<aer::PM_FP32, false>(th, st, L, B, synth, tj_leg, tj_fft);
+        case aer::PM_SPLIT:
+            return plan ? launch_table_pm<aer::PM_SPLIT, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                        : launch_table_pm<aer::PM_SPLIT, false>(th, st, L, B, synth, tj_leg, tj_fft);
         default:
             return plan ? launch_table_pm<aer::PM_MIXED, true>(th, st, L, B, synth, tj_leg, tj_fft)
                         : launch_table_pm<aer::PM_MIXED, false>(th, st, L, B, synth, tj_leg, tj_fft);
@@ -442,12 +524,9 @@ int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Ta
 // the FP32 tables exist when it needs them.
 int resolve_prec(aer_sht *h, JobSpec &s)
 {
-    if (s.prec < 0) {
-        if (h->prec != AER_PREC_FP64 && h->prec != AER_PREC_FP32)
-            return fail(AER_EINVAL, "precision not available in this build");
-        s.prec = h->prec == AER_PREC_FP32 ? 1 : 0;
-    }
-    if (s.prec) return ensure_tables32(h);
+    if (s.prec < 0) s.prec = h->prec == AER_PREC_FP32 ? 1 : (h->prec == AER_PREC_SPLIT ? 2 : 0);
+    if (s.prec == 1) return ensure_tables32(h);
+    if (s.prec == 2) return ensure_tables_split(h);
     return AER_OK;
 }
 
@@ -785,6 +864,12 @@ int aer_sht_destroy(aer_sht_t h)
     h->stage_s.release();
     h->stage_g.release();
     h->sinlat.release();
+    for (int k = 0; k < 3; ++k) {
+        h->tabS[k].release();
+        h->tabTS[k].release();
+        h->expS[k].release();
+        h->expTS[k].release();
+    }
     h->wfull.release();
     h->massg.release();
     delete h;
@@ -794,7 +879,7 @@ int aer_sht_destroy(aer_sht_t h)
 int aer_sht_set_precision(aer_sht_t h, int prec)
 {
     if (!h) return fail(AER_EINVAL, "null handle");
-    if (prec != AER_PREC_FP64 && prec != AER_PREC_FP32)
+    if (prec != AER_PREC_FP64 && prec != AER_PREC_FP32 && prec != AER_PREC_SPLIT)
         return fail(AER_EINVAL, "aer_sht_set_precision: unsupported precision");
     h->prec = prec;
     return AER_OK;
@@ -846,6 +931,7 @@ int job_to_spec(const aer_sht *h, const aer_job &j, JobSpec &s)
         case AER_JOB_PREC_HANDLE: s.prec = -1; break;
         case AER_JOB_PREC_FP64: s.prec = 0; break;
         case AER_JOB_PREC_FP32: s.prec = 1; break;
+        case AER_JOB_PREC_SPLIT: s.prec = 2; break;
         default: return fail(AER_EINVAL, "job: unknown precision");
     }
     return AER_OK;

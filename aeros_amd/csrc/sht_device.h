@@ -51,6 +51,8 @@ struct CfgDev {
     const void *tabP, *tabD, *tabQ;        // [nlm][nhp] in the compute type
     const void *tabPT, *tabDT, *tabQT;     // [m][parity][nhp][kp]: degree l = m + parity + 2k fastest
     int kp;                                // k extent of the transposed tables (multiple of 16)
+    const int *expP, *expD, *expQ;         // split form: row exponents [m][parity][nhp]
+    const int *expPT, *expDT, *expQT;      // split form: row exponents [m][parity][kp]
     const double *lfac;                    // [AER_NMODE][T+1]
     const double *wpair;                   // [nhp] w_j * 2pi/nlon (x1/2 on a self-mirrored equator row)
     const void *tw;                        // per-pass forward twiddles (generic FFT path)
@@ -72,14 +74,15 @@ struct Job {
     int mode0, mode1, mode2;
     double gsc0, gsc1;                // grid-side scale of component 0 / 1
     int accumulate;
-    int prec;                         // compute type of this job: 0 = FP64, 1 = FP32
+    int prec;                         // compute type of this job: 0 = FP64, 1 = FP32, 2 = split FP32
     int ltasks;                       // Legendre tasks of the job (its range is padded to whole blocks)
     long four_off;                    // byte offset of the job's intermediate region
 };
 
-// Kernel precision modes: every job FP64, every job FP32, or per job (the kernel
-// branches on Job::prec; a block never spans two jobs, so the branch is uniform).
-enum PrecMode { PM_FP64 = 0, PM_FP32 = 1, PM_MIXED = 2 };
+// Kernel precision modes: every job FP64, every job FP32, every job split-FP32, or
+// per job (the kernel branches on Job::prec; a block never spans two jobs, so the
+// branch is uniform).
+enum PrecMode { PM_FP64 = 0, PM_FP32 = 1, PM_SPLIT = 2, PM_MIXED = 3 };
 
 // Job table of one launch: inline in the kernel arguments (up to AER_MAXJOB jobs),
 // or in device memory (djobs != nullptr, any number of jobs; compiled plans).
@@ -117,6 +120,101 @@ template <> struct Prec<1> {
     __device__ static inline Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static inline int row(int kq, int r) { return 4 * kq + r; }
 };
+
+// Split-FP32 form: FP64 operands carried as five 10-bit integer slices,
+// x = 2^e (a1 2^-10 + a2 2^-20 + ... + a5 2^-50), with e a power-of-two scale
+// shared by every product that one MFMA step adds into one output element: per
+// row of A (over all K) and per column of B (over the step's 4 K values). With
+// |a1| <= 2^10 and |a_s| <= 2^9 after it, a level (s + t - 2 = 0..4; the products
+// with s + t > 6 are below 2^-51) sums at most 7 * 2^20 over a step, so FP32
+// accumulates it exactly; the five levels are promoted to FP64 after every step.
+// The A slices are built on the host: four as float4, the fifth in a float array
+// that follows the float4 region of the same table.
+template <> struct Prec<2> {
+    using T = double;          // B operands, staging and the Fourier intermediate
+    using Acc = d4;
+    __device__ static inline int row(int kq, int r) { return 4 * kq + r; }   // FP32 MFMA layout
+};
+
+constexpr int SPLIT_BITS = 10;
+
+__device__ __forceinline__ f4 mfma32(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One 16x16x4 step of A (this lane's slices a, rows' exponents ea of the
+// accumulator rows 4(L/16)+r) times B (this lane's value b at [L/16][L%16]).
+// Returns the step's FP64 contribution to the lane's 4 accumulator elements.
+struct Exp4 {
+    int r0, r1, r2, r3;
+};
+
+__device__ __forceinline__ d4 split_step(const float4 a, const float a5, const Exp4 ea, double b)
+{
+    double mb = fabs(b);
+    mb = fmax(mb, __shfl_xor(mb, 16));
+    mb = fmax(mb, __shfl_xor(mb, 32));   // the column's 4 K values live in lanes L ^ 16, L ^ 32
+    const int fb = mb > 0.0 ? ilogb(mb) + 1 : 0;
+    double y = ldexp(b, SPLIT_BITS - fb);
+    const double b1 = rint(y);
+    y = (y - b1) * 1024.0;
+    const double b2 = rint(y);
+    y = (y - b2) * 1024.0;
+    const double b3 = rint(y);
+    y = (y - b3) * 1024.0;
+    const double b4 = rint(y);
+    y = (y - b4) * 1024.0;
+    const double b5 = rint(y);
+    const float s1 = (float)b1, s2 = (float)b2, s3 = (float)b3, s4 = (float)b4, s5 = (float)b5;
+    const f4 z = {0.f, 0.f, 0.f, 0.f};
+    const f4 c0 = mfma32(a.x, s1, z);
+    f4 c1 = mfma32(a.x, s2, z);
+    c1 = mfma32(a.y, s1, c1);
+    f4 c2 = mfma32(a.x, s3, z);
+    c2 = mfma32(a.y, s2, c2);
+    c2 = mfma32(a.z, s1, c2);
+    f4 c3 = mfma32(a.x, s4, z);
+    c3 = mfma32(a.y, s3, c3);
+    c3 = mfma32(a.z, s2, c3);
+    c3 = mfma32(a.w, s1, c3);
+    f4 c4 = mfma32(a.x, s5, z);
+    c4 = mfma32(a.y, s4, c4);
+    c4 = mfma32(a.z, s3, c4);
+    c4 = mfma32(a.w, s2, c4);
+    c4 = mfma32(a5, s1, c4);
+    const double k = 1.0 / 1024.0;
+    const int e[4] = {ea.r0, ea.r1, ea.r2, ea.r3};
+    d4 r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double t =
+            ((((double)c4[q] * k + (double)c3[q]) * k + (double)c2[q]) * k + (double)c1[q]) * k + (double)c0[q];
+        r[q] = ldexp(t, e[q] + fb - 2 * SPLIT_BITS);
+    }
+    return r;
+}
+
+__device__ __forceinline__ Exp4 load_exp4(const int *p) { return Exp4{p[0], p[1], p[2], p[3]}; }
+
+// A slices of element idx, zero where !v: the load always reads a valid element
+// (index 0 when masked) and the mask is applied to the values, so no zero
+// aggregate is ever addressed.
+__device__ __forceinline__ float4 ld_slices(const float4 *__restrict__ p, long idx, bool v)
+{
+    const float4 a = p[v ? idx : 0];
+    return make_float4(v ? a.x : 0.f, v ? a.y : 0.f, v ? a.z : 0.f, v ? a.w : 0.f);
+}
+__device__ __forceinline__ float ld_slice5(const float *__restrict__ p, long idx, bool v)
+{
+    const float a = p[v ? idx : 0];
+    return v ? a : 0.f;
+}
+__device__ __forceinline__ double ld_masked(const double *__restrict__ p, long idx, bool v)
+{
+    const double x = p[v ? idx : 0];
+    return v ? x : 0.0;
+}
 
 // The job a task belongs to: the last job whose first task is <= task. The task
 // index is uniform over a wave, so the job index is too (readfirstlane lets the
@@ -232,6 +330,71 @@ __device__ __forceinline__ void leg_synth_body(const LaunchArgs &A, const Job &J
     const int nk0 = (C.T - m) / 2 + 1;
     const int nk1 = (C.T - m - 1 >= 0) ? (C.T - m - 1) / 2 + 1 : 0;
     const int s0n = (nk0 + 3) / 4, S = s0n + (nk1 + 3) / 4;
+    if constexpr (PREC == 2) {
+        // split form: A = float4 slices, row exponents per (table, parity, pair)
+        const long to = (long)lmb * nhp + pt * 16 + i;
+        const long ntab = (long)C.nlm * nhp;   // the fifth slices follow the float4 region
+        const float4 *__restrict__ qP = (const float4 *)C.tabP + to;
+        const float4 *__restrict__ qD = (const float4 *)C.tabD + to;
+        const float4 *__restrict__ qQ = (const float4 *)C.tabQ + to;
+        const float *__restrict__ rP = (const float *)((const float4 *)C.tabP + ntab) + to;
+        const float *__restrict__ rD = (const float *)((const float4 *)C.tabD + ntab) + to;
+        const float *__restrict__ rQ = (const float *)((const float4 *)C.tabQ + ntab) + to;
+        const long xo = (long)m * 2 * nhp + pt * 16 + 4 * kq;
+        const bool scal = J.kind == SYN_SCALAR;
+        // (every split table exists for a split job, so the loads need no guard)
+        const Exp4 eP0 = load_exp4(C.expP + xo), eP1 = load_exp4(C.expP + xo + nhp);
+        const Exp4 eD0 = load_exp4(C.expD + xo), eD1 = load_exp4(C.expD + xo + nhp);
+        const Exp4 eQ0 = load_exp4(C.expQ + xo), eQ1 = load_exp4(C.expQ + xo + nhp);
+        constexpr int KS = 4;
+        for (int c0 = 0; c0 < S; c0 += KS) {
+            float4 aA[KS], aB[KS];
+            float a5A[KS], a5B[KS];
+            double bA[KS], bB[KS];
+#pragma unroll
+            for (int u = 0; u < KS; ++u) {
+                const int st = c0 + u;
+                const int par = st < s0n ? 0 : 1;
+                const int k = (st < s0n ? st : st - s0n) * 4 + kq;
+                const bool kv = st < S && k < (par ? nk1 : nk0);
+                const int n = par + 2 * k;
+                if (scal) {
+                    aA[u] = ld_slices(qP, (long)n * nhp, kv);
+                    a5A[u] = ld_slice5(rP, (long)n * nhp, kv);
+                    bA[u] = kv ? pD[2 * n] : 0.0;
+                } else {
+                    aA[u] = ld_slices(qQ, (long)n * nhp, kv);
+                    a5A[u] = ld_slice5(rQ, (long)n * nhp, kv);
+                    bA[u] = kv ? sQ * pQ[2 * n] : 0.0;
+                    aB[u] = ld_slices(qD, (long)n * nhp, kv);
+                    a5B[u] = ld_slice5(rD, (long)n * nhp, kv);
+                    bB[u] = kv ? sD * pD[2 * n] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < KS; ++u) {
+                const int st = c0 + u;
+                if (st >= S) continue;
+                const int par = st < s0n ? 0 : 1;
+                if (scal) {
+                    const d4 c = split_step(aA[u], a5A[u], par ? eP1 : eP0, bA[u]);
+                    if (par == 0) E += c;
+                    else O += c;
+                } else {
+                    // P and Q have the parity of l-m, dP/dtheta the opposite one
+                    const d4 cq = split_step(aA[u], a5A[u], par ? eQ1 : eQ0, bA[u]);
+                    const d4 cd = split_step(aB[u], a5B[u], par ? eD1 : eD0, bB[u]);
+                    if (par == 0) {
+                        E += cq;
+                        O += cd;
+                    } else {
+                        O += cq;
+                        E += cd;
+                    }
+                }
+            }
+        }
+    } else
     for (int c0 = 0; c0 < S; c0 += KCH_S) {
         if (J.kind == SYN_SCALAR) {
             T a[KCH_S], b[KCH_S];
@@ -301,7 +464,8 @@ __global__ __launch_bounds__(256) void k_leg_synth(const LaunchArgs A)
     const Job &J = job_of<PLAN>(A, task < A.ntask ? task : A.ntask - 1, true);
     const bool active = task < A.ntask && task - J.ltask0 < J.ltasks;
     if constexpr (PM == PM_MIXED) {
-        if (J.prec) leg_synth_body<1>(A, J, task, active, wave, lane, smem_raw);
+        if (J.prec == 1) leg_synth_body<1>(A, J, task, active, wave, lane, smem_raw);
+        else if (J.prec == 2) leg_synth_body<2>(A, J, task, active, wave, lane, smem_raw);
         else leg_synth_body<0>(A, J, task, active, wave, lane, smem_raw);
     } else {
         leg_synth_body<PM>(A, J, task, active, wave, lane, smem_raw);
@@ -348,6 +512,58 @@ __device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J,
     const bool cv = item < J.nitem;
     const bool has_ek = (J.kind == ANA_VECTOR) && J.gin2 != nullptr;
     const int gpi = has_ek ? 3 : 2;  // grid fields per vector item
+    if constexpr (PREC == 2) {
+        // split form: A = float4 slices, row exponents per (table, degree row)
+        const float4 *__restrict__ qP = (const float4 *)C.tabPT;
+        const float4 *__restrict__ qD = (const float4 *)C.tabDT;
+        const float4 *__restrict__ qQ = (const float4 *)C.tabQT;
+        const long ntab = (long)(C.T + 1) * 2 * C.nhp * C.kp;   // the fifth slices follow
+        const float *__restrict__ rP = (const float *)(qP + ntab);
+        const float *__restrict__ rD = (const float *)(qD + ntab);
+        const float *__restrict__ rQ = (const float *)(qQ + ntab);
+        const long xo = (long)(m * 2 + par) * C.kp + tile * 16 + 4 * kq;
+        const bool scal = J.kind == ANA_SCALAR;
+        const Exp4 eP = load_exp4(C.expPT + xo), eD = load_exp4(C.expDT + xo), eQ = load_exp4(C.expQT + xo);
+        if (scal) {
+            const long cb = (long)(item >> 3) * mstride + mo + par * 16 + (item & 7) * 2 + part;
+            for (int p0 = 0; p0 < C.nhp; p0 += 4) {
+                const int pair = p0 + kq;
+                const long ao = abase + (long)pair * C.kp;
+                const float4 a = ld_slices(qP, ao, rv);
+                const float a5 = ld_slice5(rP, ao, rv);
+                const double b = ld_masked(four, cb + (long)pair * 32, cv);
+                acc += split_step(a, a5, eP, b);
+            }
+        } else {
+            const int gt = item * gpi, gp = gt + 1, ge = gt + 2;
+            const int eoD = 1 - par, eoQ = par;
+            const long bt = (long)(gt >> 3) * mstride + mo + (gt & 7) * 2;
+            const long bp = (long)(gp >> 3) * mstride + mo + (gp & 7) * 2;
+            const long be = (long)(ge >> 3) * mstride + mo + (ge & 7) * 2;
+            const long offD = (comp == 0 ? bt : bp) + eoD * 16 + part;
+            const double sD = comp == 0 ? 1.0 : -1.0;
+            const long offQ = (comp == 0 ? bp : bt) + eoQ * 16 + (1 - part);
+            const double sQ = part ? -1.0 : 1.0;
+            const long offE = be + par * 16 + part;
+            const bool useE = has_ek && comp == 0;
+            for (int p0 = 0; p0 < C.nhp; p0 += 4) {
+                const int pair = p0 + kq;
+                const long po = (long)pair * 32, ao = abase + (long)pair * C.kp;
+                const float4 aD = ld_slices(qD, ao, rv), aQ = ld_slices(qQ, ao, rv);
+                const float aD5 = ld_slice5(rD, ao, rv), aQ5 = ld_slice5(rQ, ao, rv);
+                const double bD = sD * ld_masked(four, offD + po, cv);
+                const double bQ = sQ * ld_masked(four, offQ + po, cv);
+                acc += split_step(aD, aD5, eD, bD);
+                acc += split_step(aQ, aQ5, eQ, bQ);
+                if (has_ek) {
+                    const float4 aP = ld_slices(qP, ao, rv);
+                    const float aP5 = ld_slice5(rP, ao, rv);
+                    const double bE = ld_masked(four, offE + po, useE && cv);
+                    acc2 += split_step(aP, aP5, eP, bE);
+                }
+            }
+        }
+    } else
     if (J.kind == ANA_SCALAR) {
         const long cb = (long)(item >> 3) * mstride + mo + par * 16 + (item & 7) * 2 + part;
         for (int p0 = 0; p0 < C.nhp; p0 += 4 * KCH) {
@@ -429,7 +645,8 @@ __global__ __launch_bounds__(256) void k_leg_anal(const LaunchArgs A)
     const Job &J = job_of<PLAN>(A, task, true);
     if (task - J.ltask0 >= J.ltasks) return;
     if constexpr (PM == PM_MIXED) {
-        if (J.prec) leg_anal_body<1>(A, J, task, lane);
+        if (J.prec == 1) leg_anal_body<1>(A, J, task, lane);
+        else if (J.prec == 2) leg_anal_body<2>(A, J, task, lane);
         else leg_anal_body<0>(A, J, task, lane);
     } else {
         leg_anal_body<PM>(A, J, task, lane);
@@ -957,10 +1174,10 @@ __global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_synth(const LaunchArgs 
     int g, sub, p0;
     const Job &J = fft_task<PLAN>(A, task, g, sub, p0);
     if constexpr (PM == PM_MIXED) {
-        if (J.prec) fft_synth_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        if (J.prec == 1) fft_synth_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
         else fft_synth_body<0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
     } else {
-        fft_synth_body<PM, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        fft_synth_body<PM == PM_FP32 ? 1 : 0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
     }
 }
 
@@ -973,10 +1190,10 @@ __global__ __launch_bounds__(AER_FFT_THREADS) void k_fft_anal(const LaunchArgs A
     int g, sub, p0;
     const Job &J = fft_task<PLAN>(A, task, g, sub, p0);
     if constexpr (PM == PM_MIXED) {
-        if (J.prec) fft_anal_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        if (J.prec == 1) fft_anal_body<1, GENERIC>(J, g, sub, p0, smem_raw, A.four);
         else fft_anal_body<0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
     } else {
-        fft_anal_body<PM, GENERIC>(J, g, sub, p0, smem_raw, A.four);
+        fft_anal_body<PM == PM_FP32 ? 1 : 0, GENERIC>(J, g, sub, p0, smem_raw, A.four);
     }
 }
 

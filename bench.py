@@ -314,6 +314,30 @@ def ensemble_config2(torch, precision, steps=50, warmup=5):
             "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
+def dynamics_step(torch, precision, steps=20, warmup=3):
+    """Whole Held-Suarez dynamics steps at T42L20 (transform halves, column terms +
+    forcing, semi-implicit leapfrog with diffusion and filter, mass fixer), eager
+    launches, all state on the device. Returns ms per step."""
+    from aeros_amd.dynamics import Stepper
+    from aeros_amd.column import vgrid_init
+    from aeros_amd.seeding import red_spectra
+    A, B = vgrid_init(NLEV, stretch_a=1.0, sigma_t=0.0, p_top=0.0)
+    st = Stepper(T, A, B, 1200.0, tau_diff=0.1 * 86400.0, eps_filter=0.06, raw_alpha=0.53,
+                 held_suarez=True, mass_fixer=True, precision=precision)
+    nlm = st.sht.nlm
+    vor = red_spectra(T, NLEV, 3, field0=0, zero_l0=True) * 1e-6
+    temp = np.zeros((NLEV, nlm), complex)
+    temp[:, 0] = 300.0 * np.sqrt(4 * np.pi)
+    lnps = np.zeros(nlm, complex)
+    lnps[0] = np.log(1.0e5) * np.sqrt(4 * np.pi)
+    st.set_state(vor, np.zeros_like(vor), temp, lnps)
+    for _ in range(warmup):
+        st.step()
+    torch.cuda.synchronize()
+    ms = time_steps(torch, st.step, steps, 0)
+    return round(ms, 4)
+
+
 def cpu_baseline(budget_s=12.0):
     """The FP64 CPU restatement (oracle/) timed in the reference's mode-B harness shape:
     an OpenMP loop over levels, each unit one scalar synthesis + one analysis."""
@@ -411,6 +435,18 @@ def main():
     assert max(graph_errs.values()) <= max(errs.values()) * 1.0001 + 1e-300, "graph replay differs from eager"
     kms, klaunch = kernel_times(sht, ts.run_batched, max(10, args.steps // 4))
 
+    # split-FP32 form (near-FP64): every job split, FP64 FFT
+    sht.set_precision("split")
+    ts.set_assembly_prec(L.AER_JOB_PREC_HANDLE)
+    ts.run_batched()
+    torch.cuda.synchronize()
+    errs_split = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
+    gsp = capture_graph(torch, sht, ts.run_batched)
+    ms_split = time_steps(torch, gsp.replay, args.steps, args.warmup)
+    sht.set_precision(args.precision)
+    if args.precision == "fp32":
+        ts.set_assembly_prec(L.AER_JOB_PREC_FP64)
+
     bytes_k, flops_k = algorithmic(sht.nlon, sht.nlat, sht.nlm, NLEV)
     set_bytes = sum(bytes_k)
     kernels = {}
@@ -461,11 +497,19 @@ def main():
         "rel_err_vs_fp64_per_field": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "rel_err_all_fp32_per_field": (None if errs_all is None else
                                        {k: float(f"{v:.3e}") for k, v in errs_all.items()}),
+        "split_fp32_sets_per_s": round(1000.0 / ms_split, 2),
+        "max_rel_err_split_vs_fp64": max(errs_split.values()),
+        "rel_err_split_per_field": {k: float(f"{v:.3e}") for k, v in errs_split.items()},
         "set_hbm_GBs": round(set_bytes / (ms * 1e-3) / 1e9, 1),
         "roofline": roof,
         "kernels": kernels,
     }
     line["ensemble_config2"] = ensemble_config2(torch, args.precision)
+    line["held_suarez_step_T42L20"] = {
+        "what": "one whole dynamics step: transform halves, column terms + HS forcing, semi-implicit "
+                "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
+        "ms_per_step_fp64": dynamics_step(torch, "fp64"),
+        "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32")}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -56,7 +56,9 @@ extern "C" {
 /* ---- arithmetic of the Legendre contraction and the longitude FFT */
 #define AER_PREC_FP64  0  /* FP64 MFMA Legendre GEMM, FP64 FFT (the plain form) */
 #define AER_PREC_FP32  1  /* FP32 MFMA Legendre GEMM, FP32 FFT */
-#define AER_PREC_SPLIT 2  /* split-FP32 MFMA Legendre GEMM (near-FP64), FP64 FFT */
+#define AER_PREC_SPLIT 2  /* split-FP32 MFMA Legendre GEMM (near-FP64), FP64 FFT: operands as
+                             five 10-bit integer slices under power-of-two scales, each
+                             MFMA step summed exactly in FP32 and promoted to FP64 */
 
 typedef struct aer_sht *aer_sht_t;
 
@@ -206,6 +208,7 @@ typedef struct aer_job {
 #define AER_JOB_PREC_HANDLE 0     /* the handle's aer_sht_set_precision setting */
 #define AER_JOB_PREC_FP64   1
 #define AER_JOB_PREC_FP32   2
+#define AER_JOB_PREC_SPLIT  3
 
 /* All jobs must be of one direction (all AER_JOB_SYNTH_* or all AER_JOB_ANAL_*);
  * at most AER_MAX_JOBS jobs per call. Asynchronous on the handle's stream. */

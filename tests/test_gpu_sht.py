@@ -224,3 +224,32 @@ def test_fp32_error_is_stated(torch):
         assert es < 1e-6 and ea < 1e-6
     finally:
         s.set_precision("fp64")
+
+
+@pytest.mark.parametrize("T,nlev", [(21, 8), (42, 20), (106, 3)])
+def test_split_error_is_stated(torch, T, nlev):
+    """Split-FP32 form (FP32 MFMA on exact 10-bit integer slices, FP64 promotion per
+    step, FP64 FFT): every transform kind against the FP64 oracle. The bound is the
+    measured error with a margin; the slices carry 50 bits, so the representation
+    error is 2^-51 of each row/column scale."""
+    s, o = sht(T, "split"), orc(T)
+    try:
+        q = red_spectra(T, nlev, config=8)
+        t = red_spectra(T, nlev, config=9, zero_l0=True)
+        gq = s.synthesis(dev(torch, q)).cpu().numpy()
+        gin = np.stack([grid_from(T, 3000 + k) for k in range(nlev)])
+        qa = s.analysis(dev(torch, gin)).cpu().numpy()
+        vt, vp = (x.cpu().numpy() for x in s.synthesis_vec(dev(torch, t), dev(torch, q)))
+        ss, tt = (x.cpu().numpy() for x in s.analysis_vec(dev(torch, gin), dev(torch, gin[::-1].copy())))
+        gt, gp = (x.cpu().numpy() for x in s.gradient(dev(torch, q)))
+        errs = {
+            "synth": max(rel(gq[k], o.synth(q[k])) for k in range(nlev)),
+            "anal": max(rel(qa[k], o.anal(gin[k])) for k in range(nlev)),
+            "synth_vec": max(max(rel(a, b) for a, b in zip((vt[k], vp[k]), o.synth_vec(t[k], q[k]))) for k in range(nlev)),
+            "anal_vec": max(max(rel(a, b) for a, b in zip((ss[k], tt[k]), o.anal_vec(gin[k], gin[::-1][k]))) for k in range(nlev)),
+            "gradient": max(max(rel(a, b) for a, b in zip((gt[k], gp[k]), o.synth_grad(q[k]))) for k in range(nlev)),
+        }
+        print(f"split T{T}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+        assert max(errs.values()) < 1e-12, errs
+    finally:
+        s.set_precision("fp64")

@@ -159,3 +159,14 @@ def test_planted_mistakes_are_caught():
     # geopotential of the wrong level (shifted by one)
     t = o.tendency_spectral(ae, an, np.roll(ek, 1, axis=0), dtdt, dps)
     assert rest_errors(ek, t["vor"], t["div"], t["temp"], t["lnps"])["div"] > 1e-3
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(stretch_a=1.0, sigma_t=0.0, p_top=0.0), dict(sigma_t=0.35, p_top=500.0)])
+def test_product_vgrid_generator_matches_oracle(kw):
+    """The host-side generator of aeros_amd (used by callers without an A/B table)
+    against the oracle's restatement."""
+    from aeros_amd.column import vgrid_init
+    for nlev in (12, 20, 31):
+        a, b = vgrid_init(nlev, **kw)
+        A, B = O.vgrid_init(nlev, **kw)
+        assert np.abs(a - A).max() <= 1e-9 and np.abs(b - B).max() <= 1e-15

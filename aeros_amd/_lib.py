@@ -86,6 +86,11 @@ SIGNATURES["aer_run_jobs_d"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob)])
 SIGNATURES["aer_plan_create"] = (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(AerJob), ctypes.POINTER(_vp)])
 SIGNATURES["aer_plan_run"] = (_c_int, [_vp, _vp])
 SIGNATURES["aer_plan_destroy"] = (_c_int, [_vp])
+SIGNATURES["aer_synth_adj_d"] = (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long])
+SIGNATURES["aer_anal_adj_d"] = (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long])
+SIGNATURES["aer_synth_vec_adj_d"] = (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _vp, _c_long])
+SIGNATURES["aer_anal_vec_adj_d"] = (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _vp, _c_long])
+SIGNATURES["aer_synth_grad_adj_d"] = (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _c_long])
 SIGNATURES["aer_vgrid_create"] = (_c_int, [_c_int, _dp, _dp, ctypes.POINTER(_vp)])
 SIGNATURES["aer_vgrid_destroy"] = (_c_int, [_vp])
 SIGNATURES["aer_column_terms_d"] = (_c_int, [_vp, _vp, _c_int] + [_vp] * 16)

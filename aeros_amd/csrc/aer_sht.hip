@@ -387,6 +387,7 @@ struct JobSpec {
     double gsc0 = 1.0, gsc1 = 1.0;
     int accumulate = 0;
     int prec = -1;          // compute type: 0 = FP64, 1 = FP32; -1 = the handle's (resolved at launch)
+    int adj = 0;            // transpose of the opposite transform (see aer::Job::adj)
 };
 
 // Job table of one direction: jobs on handles hs[k] (one truncation each), their
@@ -440,6 +441,7 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         J.mode0 = s.mode0; J.mode1 = s.mode1; J.mode2 = s.mode2;
         J.gsc0 = s.gsc0; J.gsc1 = s.gsc1;
         J.accumulate = s.accumulate;
+        J.adj = s.adj;
         J.four_off = (long)B.four_bytes;
         B.four_bytes += ((size_t)J.ngroup * (C.T + 1) * C.nhp * 32 * (cxb / 2) + 255) / 256 * 256;
         J.ltask0 = B.ltask;
@@ -1155,6 +1157,67 @@ int aer_synth_grad_d(aer_sht_t h, int nfield, const double *slm, long qstride,
     JobSpec s{aer::SYN_GRAD, nfield};
     s.sin0 = slm; s.sstride = qstride; s.gout0 = gt; s.gout1 = gp; s.gstride = gstride;
     return run_jobs(h, &s, 1, true);
+}
+
+// ---- adjoint (transpose) transforms
+
+int aer_synth_adj_d(aer_sht_t h, int nfield, const double *vr, long gstride, double *qlm, long qstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::ANA_SCALAR, nfield};
+    s.gin0 = vr; s.gstride = gstride; s.sout0 = qlm; s.sstride = qstride;
+    s.adj = 1;
+    return run_jobs(h, &s, 1, false);
+}
+
+int aer_anal_adj_d(aer_sht_t h, int nfield, const double *qlm, long qstride, double *vr, long gstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_SCALAR, nfield};
+    s.sin0 = qlm; s.sstride = qstride; s.gout0 = vr; s.gstride = gstride;
+    s.adj = 1;
+    return run_jobs(h, &s, 1, true);
+}
+
+int aer_synth_vec_adj_d(aer_sht_t h, int nfield, const double *vt, const double *vp, long gstride,
+                        double *slm, double *tlm, long qstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    if (!slm || !tlm) return fail(AER_EINVAL, "aer_synth_vec_adj_d: missing output");
+    JobSpec s{aer::ANA_VECTOR, nfield};
+    s.gin0 = vt; s.gin1 = vp; s.gstride = gstride;
+    s.sout0 = slm; s.sout1 = tlm; s.sstride = qstride;
+    s.adj = 1;
+    return run_jobs(h, &s, 1, false);
+}
+
+int aer_anal_vec_adj_d(aer_sht_t h, int nfield, const double *slm, const double *tlm, long qstride,
+                       double *vt, double *vp, long gstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    JobSpec s{aer::SYN_VECTOR, nfield};
+    s.sin0 = slm; s.sin1 = tlm; s.sstride = qstride;
+    s.gout0 = vt; s.gout1 = vp; s.gstride = gstride;
+    s.mode0 = s.mode1 = aer::M_INV_LL1;
+    s.adj = 1;
+    return run_jobs(h, &s, 1, true);
+}
+
+int aer_synth_grad_adj_d(aer_sht_t h, int nfield, const double *gt, const double *gp, long gstride,
+                         double *slm, long qstride)
+{
+    int rc = check_batch(h, nfield, qstride, gstride);
+    if (rc) return rc;
+    if (!slm) return fail(AER_EINVAL, "aer_synth_grad_adj_d: missing output");
+    JobSpec s{aer::ANA_VECTOR, nfield};
+    s.gin0 = gt; s.gin1 = gp; s.gstride = gstride;
+    s.sout0 = slm; s.sout1 = nullptr; s.sstride = qstride;
+    s.adj = 1;
+    return run_jobs(h, &s, 1, false);
 }
 
 // ---- aeros_vordiv

@@ -75,6 +75,9 @@ struct Job {
     double gsc0, gsc1;                // grid-side scale of component 0 / 1
     int accumulate;
     int prec;                         // compute type of this job: 0 = FP64, 1 = FP32, 2 = split FP32
+    int adj;                          // transpose of the opposite transform: synthesis jobs halve
+                                      // m > 0 and weight rows by w_j 2pi/nlon; analysis jobs use
+                                      // unit row weights and double m > 0
     int ltasks;                       // Legendre tasks of the job (its range is padded to whole blocks)
     long four_off;                    // byte offset of the job's intermediate region
 };
@@ -620,6 +623,7 @@ __device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J,
     }
     if (!cv) return;
     double *out = (J.kind == ANA_VECTOR && comp == 1) ? J.sout1 : J.sout0;
+    if (!out) return;   // a vector job may leave its T output out (adjoint of the gradient)
     out += (long)item * J.sstride;
     const double *fa = C.lfac + ((J.kind == ANA_VECTOR && comp == 1) ? J.mode1 : J.mode0) * (C.T + 1);
     const double *fe = C.lfac + J.mode2 * (C.T + 1);
@@ -923,6 +927,8 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
     T2 *bufA = smem, *bufB = smem + PL, *tw = smem + 2 * PL;
     const T2 *twg = (const T2 *)C.twN;
     const int M = C.T;
+    const int adj = J.adj;
+    const double *wpair = C.wpair;
     // Every global load of the block is issued before the first wait: the
     // intermediate values (T + 1 <= N/2 since the grid has nlon > 2T) and the
     // twiddles go to registers, then to LDS.
@@ -957,7 +963,8 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
         const int rp = idx % ROWS, m = idx / ROWS;
         if (m <= M) {
             T2 zm, zc;
-            synth_pack(er[u], ei[u], orr[u], oi[u], m, zm, zc);
+            const T hf = (adj && m > 0) ? T(0.5) : T(1);
+            synth_pack(hf * er[u], hf * ei[u], hf * orr[u], hf * oi[u], m, zm, zc);
             bufA[pidx(rp * N + m)] = zm;
             if (m > 0) bufA[pidx(rp * N + N - m)] = zc;
         }
@@ -977,6 +984,7 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
         double *out = syn_out(J, gf, sc);
         const T2 z = res[pidx(idx)];
         const int js = C.nlat - 1 - pair;
+        if (adj) sc *= wpair[pair] * (js == pair ? 2.0 : 1.0);   // w_j 2pi/nlon (the equator's half undone)
         out[(long)pair * N + i] = sc * (double)z.x;
         if (js != pair) out[(long)js * N + i] = sc * (double)z.y;
     }
@@ -1021,7 +1029,7 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     __syncthreads();
     // job fields read before the global stores below (kernel-argument memory is
     // then never re-read after a store that might alias it)
-    const int M = C.T, nh = C.nh, nhp = C.nhp, ngf = J.ngf;
+    const int M = C.T, nh = C.nh, nhp = C.nhp, ngf = J.ngf, adj = J.adj, nlat = C.nlat;
     const double *wpair = C.wpair;
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
     for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
@@ -1030,7 +1038,8 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
         if (pair >= nhp) continue;
         T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
         if (pair < nh && g * 8 + gfl < ngf) {
-            anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(wpair[pair]), o);
+            const double w = adj ? ((nlat - 1 - pair == pair) ? 0.5 : 1.0) * (m > 0 ? 2.0 : 1.0) : wpair[pair];
+            anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(w), o);
         } else {
             o[0] = o[1] = o[16] = o[17] = T(0);
         }
@@ -1055,7 +1064,8 @@ __device__ __forceinline__ void fft_synth_generic(const Job &J, const CfgDev &C,
             if (m >= 0) {
                 T2 zm, zc;
                 const T *b = four + ((long)m * C.nhp + pair) * 32 + gfl * 2;
-                synth_pack(b[0], b[1], b[16], b[17], m, zm, zc);
+                const T hf = (J.adj && m > 0) ? T(0.5) : T(1);
+                synth_pack(hf * b[0], hf * b[1], hf * b[16], hf * b[17], m, zm, zc);
                 z = (k <= M) ? zm : zc;
             }
         }
@@ -1063,6 +1073,8 @@ __device__ __forceinline__ void fft_synth_generic(const Job &J, const CfgDev &C,
     }
     __syncthreads();
     const T2 *res = fft_rows<+1, T>(bufA, bufB, rows, C);
+    const int adj = J.adj;
+    const double *wpair = C.wpair;
     for (int idx = threadIdx.x; idx < rows * N; idx += blockDim.x) {
         const int rp = idx / N, i = idx - rp * N;
         const int gfl = sub * C.fb + rp / C.pb, pair = p0 + rp % C.pb, gf = g * 8 + gfl;
@@ -1071,6 +1083,7 @@ __device__ __forceinline__ void fft_synth_generic(const Job &J, const CfgDev &C,
         double *out = syn_out(J, gf, sc);
         const T2 z = res[idx];
         const int js = C.nlat - 1 - pair;
+        if (adj) sc *= wpair[pair] * (js == pair ? 2.0 : 1.0);
         out[(long)pair * N + i] = sc * (double)z.x;
         if (js != pair) out[(long)js * N + i] = sc * (double)z.y;
     }
@@ -1095,7 +1108,7 @@ __device__ __forceinline__ void fft_anal_generic(const Job &J, const CfgDev &C, 
     }
     __syncthreads();
     const T2 *res = fft_rows<-1, T>(bufA, bufB, rows, C);
-    const int nh = C.nh, nhp = C.nhp, ngf = J.ngf, fbk = C.fb, pbk = C.pb;
+    const int nh = C.nh, nhp = C.nhp, ngf = J.ngf, fbk = C.fb, pbk = C.pb, adj = J.adj, nlat = C.nlat;
     const double *wpair = C.wpair;
     for (int idx = threadIdx.x; idx < rows * (M + 1); idx += blockDim.x) {
         const int rp = idx / (M + 1), m = idx - rp * (M + 1);
@@ -1103,7 +1116,8 @@ __device__ __forceinline__ void fft_anal_generic(const Job &J, const CfgDev &C, 
         if (pair >= nhp) continue;
         T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
         if (pair < nh && g * 8 + gfl < ngf)
-            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)], T(wpair[pair]), o);
+            anal_unpack(res[rp * N + m], res[rp * N + (m == 0 ? 0 : N - m)],
+                        T(adj ? ((nlat - 1 - pair == pair) ? 0.5 : 1.0) * (m > 0 ? 2.0 : 1.0) : wpair[pair]), o);
         else
             o[0] = o[1] = o[16] = o[17] = T(0);
     }

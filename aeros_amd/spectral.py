@@ -200,6 +200,49 @@ class AerosSht:
         check(lib().aer_spat_to_SHsphtor(self._h, vt.ctypes.data, vp.ctypes.data, s.ctypes.data, t.ctypes.data))
         return s, t
 
+    # ------------------------------------------------------------ adjoints (device tensors only)
+    def synthesis_adjoint(self, g):
+        """synthesis^T: grid (..., nlat, nlon) -> spectral (..., nlm)."""
+        g = self._dev_grid(g)
+        out = self._new_spec(g, g.shape[:-2])
+        self._bind_stream()
+        check(lib().aer_synth_adj_d(self._h, self._n(g, 2), g.data_ptr(), self.nlon * self.nlat,
+                                    out.data_ptr(), 2 * self.nlm))
+        return out
+
+    def analysis_adjoint(self, q):
+        """analysis^T: spectral (..., nlm) -> grid (..., nlat, nlon)."""
+        q = self._dev_spec(q)
+        out = self._new_grid(q, q.shape[:-1])
+        self._bind_stream()
+        check(lib().aer_anal_adj_d(self._h, self._n(q, 1), q.data_ptr(), 2 * self.nlm,
+                                   out.data_ptr(), self.nlon * self.nlat))
+        return out
+
+    def synthesis_vec_adjoint(self, vt, vp):
+        vt, vp = self._dev_grid(vt), self._dev_grid(vp)
+        s, t = self._new_spec(vt, vt.shape[:-2]), self._new_spec(vt, vt.shape[:-2])
+        self._bind_stream()
+        check(lib().aer_synth_vec_adj_d(self._h, self._n(vt, 2), vt.data_ptr(), vp.data_ptr(),
+                                        self.nlon * self.nlat, s.data_ptr(), t.data_ptr(), 2 * self.nlm))
+        return s, t
+
+    def analysis_vec_adjoint(self, s, t):
+        s, t = self._dev_spec(s), self._dev_spec(t)
+        vt, vp = self._new_grid(s, s.shape[:-1]), self._new_grid(s, s.shape[:-1])
+        self._bind_stream()
+        check(lib().aer_anal_vec_adj_d(self._h, self._n(s, 1), s.data_ptr(), t.data_ptr(), 2 * self.nlm,
+                                       vt.data_ptr(), vp.data_ptr(), self.nlon * self.nlat))
+        return vt, vp
+
+    def gradient_adjoint(self, gt, gp):
+        gt, gp = self._dev_grid(gt), self._dev_grid(gp)
+        s = self._new_spec(gt, gt.shape[:-2])
+        self._bind_stream()
+        check(lib().aer_synth_grad_adj_d(self._h, self._n(gt, 2), gt.data_ptr(), gp.data_ptr(),
+                                         self.nlon * self.nlat, s.data_ptr(), 2 * self.nlm))
+        return s
+
     def gradient(self, q):
         """Unit-sphere (dF/dtheta, (1/sin) dF/dphi) (SHsph_to_spat, src/aeros_spectral.f90:560-584)."""
         if _is_torch(q):

@@ -229,6 +229,27 @@ int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_pl
 int aer_plan_run(aer_plan_t plan, void *stream);
 int aer_plan_destroy(aer_plan_t plan);
 
+/* ---- adjoint (transpose) transforms, for tangent-linear / adjoint models (no
+ * reference counterpart). Each is the exact transpose of the named transform in
+ * the real inner products sum(re*re + im*im) over the nlm coefficients and
+ * sum over the nlon*nlat grid points (unweighted):
+ *   aer_synth_adj_d      spectral = synthesis^T (grid)
+ *   aer_anal_adj_d       grid     = analysis^T (spectral)
+ *   aer_synth_vec_adj_d  (S, T)   = vector synthesis^T (V_theta, V_phi)
+ *   aer_anal_vec_adj_d   (V_theta, V_phi) = vector analysis^T (S, T)
+ *   aer_synth_grad_adj_d S        = gradient synthesis^T (g_theta, g_phi)
+ * They run the forward kernels with unit row weights and the m > 0 factor 2 in the
+ * analysis direction, and with the m > 0 coefficients halved and the rows weighted
+ * by w_j 2 pi/nlon in the synthesis direction. Strides as for the batched calls. */
+int aer_synth_adj_d(aer_sht_t cfg, int nfield, const double *vr, long gstride, double *qlm, long qstride);
+int aer_anal_adj_d(aer_sht_t cfg, int nfield, const double *qlm, long qstride, double *vr, long gstride);
+int aer_synth_vec_adj_d(aer_sht_t cfg, int nfield, const double *vt, const double *vp, long gstride,
+                        double *slm, double *tlm, long qstride);
+int aer_anal_vec_adj_d(aer_sht_t cfg, int nfield, const double *slm, const double *tlm, long qstride,
+                       double *vt, double *vp, long gstride);
+int aer_synth_grad_adj_d(aer_sht_t cfg, int nfield, const double *gt, const double *gp, long gstride,
+                         double *slm, long qstride);
+
 /* ---- grid-space column work between the transform halves of a step.
  *
  * Vertical coordinate: the hybrid coefficients of the reference's

@@ -552,7 +552,11 @@ int run_jobs(aer_sht *h, JobSpec *specs, int njob, bool synth)
     L.njob = (int)B.jobs.size();
     L.four = h->four.p;
     L.djobs = nullptr;
-    for (int k = 0; k < L.njob; ++k) L.job[k] = B.jobs[k];
+    for (int k = 0; k < L.njob; ++k) {
+        L.job[k] = B.jobs[k];
+        L.lstart[k] = B.jobs[k].ltask0;
+        L.fstart[k] = B.jobs[k].ftask0;
+    }
     return launch_table(h, h->stream, L, B, synth);
 }
 
@@ -1785,7 +1789,7 @@ int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *m
     sp.prec = 0;   // global mass sums stay FP64
     int rc = run_jobs(h, &sp, 1, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(aer::k_mass, dim3(1), dim3(256), (size_t)h->nlat * sizeof(double), h->stream,
+    hipLaunchKernelGGL(aer::k_mass, dim3(1), dim3(256), (size_t)5 * h->nlat * sizeof(double), h->stream,
                        (const double *)h->massg.p, h->nlon, h->nlat, (const double *)h->wfull.p, 9.80665,
                        fix ? 1 : 0, mass_target, h->c00, lnps, mass_out, lnr_out);
     HIP_TRY(hipGetLastError());

@@ -91,6 +91,7 @@ enum PrecMode { PM_FP64 = 0, PM_FP32 = 1, PM_SPLIT = 2, PM_MIXED = 3 };
 // or in device memory (djobs != nullptr, any number of jobs; compiled plans).
 struct LaunchArgs {
     int njob, ntask;
+    int lstart[AER_MAXJOB], fstart[AER_MAXJOB];   // first Legendre / FFT task of each inline job
     int kmax;               // max T+1 over the jobs (LDS staging of the Legendre synthesis)
     void *four;
     const Job *djobs;       // device job table (compiled plans)
@@ -234,9 +235,14 @@ __device__ __forceinline__ const Job &job_of(const LaunchArgs &A, int task, bool
     // Inline table: indexed straight in the kernel-argument segment (LaunchArgs is
     // the only kernel argument, at offset 0), not through the by-value parameter,
     // whose dynamically indexed use would keep a private copy of all of it.
+    // The task starts sit together at the front of the arguments, so they load at
+    // once and the job index is a count, not a chain of dependent loads.
     const LaunchArgs *K = (const LaunchArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const int *st = legendre ? K->lstart : K->fstart;
+    const int nj = K->njob;
     int jb = 0;
-    while (jb + 1 < K->njob && task >= (legendre ? K->job[jb + 1].ltask0 : K->job[jb + 1].ftask0)) ++jb;
+#pragma unroll
+    for (int k = 1; k < AER_MAXJOB; ++k) jb += (k < nj && task >= st[k]) ? 1 : 0;
     jb = __builtin_amdgcn_readfirstlane(jb);
     return K->job[jb];
 }

@@ -236,11 +236,21 @@ __global__ __launch_bounds__(256) void k_mass(const double *lnps_g, int nlon, in
                                               double grav, int fix, double target, double c00,
                                               double *lnps, double *mass_out, double *lnr_out)
 {
-    extern __shared__ double rows[];
+    extern __shared__ double rows[];   // nlat row totals, then 4 partials per row
+    double *part = rows + nlat;
     const double dlon = 2.0 * 3.14159265358979323846264338327950288 / (double)nlon;
+    // four threads per row, each a contiguous quarter; the quarters are added in order
+    const int qn = (nlon + 3) / 4;
+    for (int t = threadIdx.x; t < 4 * nlat; t += blockDim.x) {
+        const int j = t >> 2, q = t & 3;
+        const int i0 = q * qn, i1 = min(nlon, i0 + qn);
+        double p = 0.0;
+        for (int i = i0; i < i1; ++i) p = p + exp(lnps_g[(long)j * nlon + i]);
+        part[t] = p;
+    }
+    __syncthreads();
     for (int j = threadIdx.x; j < nlat; j += blockDim.x) {
-        double row = 0.0;
-        for (int i = 0; i < nlon; ++i) row = row + exp(lnps_g[(long)j * nlon + i]);
+        const double row = ((part[4 * j] + part[4 * j + 1]) + part[4 * j + 2]) + part[4 * j + 3];
         rows[j] = w[j] * row * dlon;
     }
     __syncthreads();

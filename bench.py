@@ -364,8 +364,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--only", choices=["set", "ensemble"], default=None,
-                    help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms) or only the ensemble")
+    ap.add_argument("--only", choices=["set", "ensemble", "step"], default=None,
+                    help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms), only the "
+                         "ensemble, or only whole Held-Suarez dynamics steps")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -384,6 +385,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 8, "launches": 5}))
+        return
+    if args.only == "step":
+        print(json.dumps({"held_suarez_step_ms": dynamics_step(torch, args.precision, args.steps, args.warmup),
+                          "precision": args.precision}))
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup)}))

@@ -20,6 +20,8 @@ AER_SHT_QUICK_INIT = 0x40
 AER_SHT_LOAD_SAVE_CFG = 0x80
 
 AER_PREC_FP64, AER_PREC_FP32, AER_PREC_SPLIT = 0, 1, 2
+AER_PATH_TILED, AER_PATH_STAGED = 0, 1
+AER_NKERNEL_KIND = 5
 
 # Every symbol include/aer_sht.h declares, with (restype, argtypes).
 _c_int, _c_long, _c_double, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
@@ -46,6 +48,7 @@ SIGNATURES = {
     "aer_sht_destroy": (_c_int, [_vp]),
     "aer_sht_set_precision": (_c_int, [_vp, _c_int]),
     "aer_sht_set_stream": (_c_int, [_vp, _vp]),
+    "aer_sht_set_path": (_c_int, [_vp, _c_int]),
     "aer_sht_sync": (_c_int, [_vp]),
     "aer_sht_timing": (_c_int, [_vp, _c_int]),
     "aer_sht_timing_read": (_c_int, [_vp, _dp, _ip]),
@@ -86,6 +89,9 @@ SIGNATURES["aer_run_jobs_d"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob)])
 SIGNATURES["aer_plan_create"] = (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(AerJob), ctypes.POINTER(_vp)])
 SIGNATURES["aer_plan_run"] = (_c_int, [_vp, _vp])
 SIGNATURES["aer_plan_destroy"] = (_c_int, [_vp])
+AER_TUNE_NCAND = 4
+SIGNATURES["aer_autotune_jobs"] = (_c_int, [_vp, _c_int, ctypes.POINTER(AerJob), _c_int, _dp, _ip])
+SIGNATURES["aer_plan_autotune"] = (_c_int, [_vp, _vp, _c_int, _dp, _ip])
 SIGNATURES["aer_synth_adj_d"] = (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long])
 SIGNATURES["aer_anal_adj_d"] = (_c_int, [_vp, _c_int, _vp, _c_long, _vp, _c_long])
 SIGNATURES["aer_synth_vec_adj_d"] = (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _vp, _vp, _c_long])

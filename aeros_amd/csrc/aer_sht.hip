@@ -3,11 +3,13 @@
 // FFT plans, and the job builder that turns each API call into one launch per
 // stage (see sht_device.h for the kernels).
 #include "sht_device.h"
+#include "fused_device.h"
 #include "column_device.h"
 #include "step_device.h"
 #include "../../include/aer_sht.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -64,13 +66,18 @@ struct aer_sht {
     double eps = 0.0;
     bool grid_set = false;
     int prec = AER_PREC_FP64;
+    int path_syn = AER_PATH_STAGED;        // kernel path of synthesis launches (aer_sht_set_path, autotune)
+    int path_ana = AER_PATH_STAGED;        // kernel path of analysis launches
+    int band_kb = 64;                      // tiled synthesis: staging band budget in KiB (autotune)
     hipStream_t stream = nullptr;
     std::vector<double> ct, w;             // nodes (cos colatitude) N->S, weights
     std::vector<int> li, mi;
     std::vector<int> radix, twoff;
     // device data
-    DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
+    DevBuf tab64[3], tab32[3], tabT64[3], tabT32[3], lfac, lfacm, wpair, tw64, tw32, twN64, twN32, four, stage_s, stage_g;
     DevBuf tabS[3], tabTS[3], expS[3], expTS[3];   // split form: float4 slices and row exponents
+    DevBuf tabZ64[3], tabZ32[3], tabY64[3], tabY32[3];   // tiled path layouts (tile_table; quad_table / group_table)
+    int nq = 1;                             // row quads of the tiled analysis tables
     DevBuf sinlat;                          // cos(colatitude) per row, for the column kernel
     DevBuf wfull, massg;                    // Gauss weights (nlat) and a grid for the mass sums
     double c00 = 0.0;                       // (0,0) coefficient of the constant field 1
@@ -185,6 +192,56 @@ std::vector<double> transpose_table(const std::vector<double> &X, int T, int nhp
     return Y;
 }
 
+// Tiled-synthesis copy of a [lm][nhp] table: [tile][lm][4], 4 latitude pairs per tile,
+// so the slice of one tile is contiguous.
+std::vector<double> tile_table(const std::vector<double> &X, int nlm, int nhp, int ntile)
+{
+    std::vector<double> Y((size_t)ntile * nlm * 4, 0.0);
+    for (int t = 0; t < ntile; ++t)
+        for (int lm = 0; lm < nlm; ++lm)
+            for (int i = 0; i < 4; ++i) Y[((size_t)t * nlm + lm) * 4 + i] = X[(size_t)lm * nhp + 4 * t + i];
+    return Y;
+}
+
+// Tiled-analysis copy: [m][parity][row quad rq][pair < nh4][4] holding the degree
+// l = m + parity + 2(4 rq + i) in slot i, zero where l > T.
+std::vector<double> quad_table(const std::vector<double> &X, int T, int nhp, int nh4, int nq)
+{
+    std::vector<double> Y((size_t)(T + 1) * 2 * nq * nh4 * 4, 0.0);
+    for (int m = 0; m <= T; ++m) {
+        const int lmb = m * (2 * T + 3 - m) / 2;
+        for (int par = 0; par < 2; ++par)
+            for (int rq = 0; rq < nq; ++rq)
+                for (int p = 0; p < nh4; ++p)
+                    for (int i = 0; i < 4; ++i) {
+                        const int l = m + par + 2 * (4 * rq + i);
+                        if (l > T) continue;
+                        Y[((((size_t)m * 2 + par) * nq + rq) * nh4 + p) * 4 + i] = X[(size_t)(lmb + l - m) * nhp + p];
+                    }
+    }
+    return Y;
+}
+
+// FP32 tiled-analysis copy: [m][parity][group G of 4 row quads][pair < nh4][16] holding
+// the degree l = m + parity + 2(16 G + s) in slot s, zero where l > T.
+std::vector<double> group_table(const std::vector<double> &X, int T, int nhp, int nh4, int nq)
+{
+    const int nG = (nq + 3) / 4;
+    std::vector<double> Y((size_t)(T + 1) * 2 * nG * nh4 * 16, 0.0);
+    for (int m = 0; m <= T; ++m) {
+        const int lmb = m * (2 * T + 3 - m) / 2;
+        for (int par = 0; par < 2; ++par)
+            for (int G = 0; G < nG; ++G)
+                for (int p = 0; p < nh4; ++p)
+                    for (int sl = 0; sl < 16; ++sl) {
+                        const int l = m + par + 2 * (16 * G + sl);
+                        if (l > T) continue;
+                        Y[((((size_t)m * 2 + par) * nG + G) * nh4 + p) * 16 + sl] = X[(size_t)(lmb + l - m) * nhp + p];
+                    }
+    }
+    return Y;
+}
+
 int upload(DevBuf &b, const void *src, size_t bytes)
 {
     int rc = b.ensure(bytes);
@@ -224,6 +281,10 @@ int ensure_tables32(aer_sht *h)
         auto ft = convert<float>(transpose_table(*src[k], h->lmax, h->nhp, h->kp));
         rc = upload(h->tabT32[k], ft.data(), ft.size() * sizeof(float));
         if (rc) return rc;
+        auto fz = convert<float>(tile_table(*src[k], h->nlm, h->nhp, (h->nh + 3) / 4));
+        if ((rc = upload(h->tabZ32[k], fz.data(), fz.size() * sizeof(float)))) return rc;
+        auto fy = convert<float>(group_table(*src[k], h->lmax, h->nhp, (h->nh + 3) / 4 * 4, h->nq));
+        if ((rc = upload(h->tabY32[k], fy.data(), fy.size() * sizeof(float)))) return rc;
     }
     auto tw32 = convert<float>(h->tw_host);
     int rc = upload(h->tw32, tw32.data(), tw32.size() * sizeof(float));
@@ -330,7 +391,17 @@ aer::CfgDev cfg_dev(const aer_sht *h, int prec)
     c.tabDT = tabT[1].p;
     c.tabQT = tabT[2].p;
     c.kp = h->kp;
+    const DevBuf *tz = (prec == AER_PREC_FP32) ? h->tabZ32 : h->tabZ64;
+    const DevBuf *ty = (prec == AER_PREC_FP32) ? h->tabY32 : h->tabY64;
+    c.tabZP = tz[0].p;
+    c.tabZD = tz[1].p;
+    c.tabZQ = tz[2].p;
+    c.tabYP = ty[0].p;
+    c.tabYD = ty[1].p;
+    c.tabYQ = ty[2].p;
+    c.nq = h->nq;
     c.lfac = (const double *)h->lfac.p;
+    c.lfacm = (const double *)h->lfacm.p;
     c.wpair = (const double *)h->wpair.p;
     c.tw = (prec == AER_PREC_FP32) ? h->tw32.p : h->tw64.p;
     c.twN = (prec == AER_PREC_FP32) ? h->twN32.p : h->twN64.p;
@@ -402,6 +473,8 @@ struct Table {
     int kmax = 1;           // max T+1 over the jobs
     int pm = aer::PM_FP64;  // kernel precision mode
     size_t cxmax = 16;      // largest complex element size among the jobs
+    bool tiled = false;     // tiled path (fused synthesis; 4x4-pair analysis FFT + per-m-pair Legendre)
+    size_t lds_leg = 0;     // tiled analysis Legendre kernel's dynamic LDS
 };
 
 void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
@@ -463,6 +536,108 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
     B.cxmax = cxmax ? cxmax : 16;
 }
 
+// Job table of the tiled path. Returns false when some job cannot take it (split
+// precision, a grid length without a compile-time FFT, or LDS beyond the CU's).
+bool build_table_tiled(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
+{
+    B = Table{};
+    B.tiled = true;
+    bool any64 = false, any32 = false;
+    for (int k = 0; k < njob; ++k) {
+        const JobSpec &s = specs[k];
+        if (s.nitem <= 0) continue;
+        const aer_sht *h = hs[k];
+        if (s.prec != 0 && s.prec != 1) return false;
+        if ((synth ? h->path_syn : h->path_ana) != AER_PATH_TILED || !aer::fft_fixed(h->nlon)) return false;
+        // the synthesis stages coefficients as 16-byte complex loads
+        if (synth && ((s.sstride & 1) || ((uintptr_t)s.sin0 & 15) || ((uintptr_t)s.sin1 & 15))) return false;
+        const aer::CfgDev C = cfg_dev(h, s.prec == 1 ? AER_PREC_FP32 : AER_PREC_FP64);
+        const int csz = s.prec == 1 ? (int)sizeof(float) : (int)sizeof(double);
+        (s.prec == 1 ? any32 : any64) = true;
+        aer::Job J{};
+        J.c = C;
+        J.kind = s.kind;
+        J.nitem = s.nitem;
+        J.prec = s.prec;
+        switch (s.kind) {
+            case aer::SYN_SCALAR: case aer::ANA_SCALAR: J.ngf = s.nitem; break;
+            case aer::SYN_VECTOR: case aer::SYN_GRAD: J.ngf = 2 * s.nitem; break;
+            default: J.ngf = (s.gin2 ? 3 : 2) * s.nitem; break;
+        }
+        J.sstride = s.sstride;
+        J.gstride = s.gstride;
+        J.sin0 = s.sin0; J.sin1 = s.sin1;
+        J.sout0 = s.sout0; J.sout1 = s.sout1;
+        J.gout0 = s.gout0; J.gout1 = s.gout1;
+        J.gin0 = s.gin0; J.gin1 = s.gin1; J.gin2 = s.gin2;
+        J.mode0 = s.mode0; J.mode1 = s.mode1; J.mode2 = s.mode2;
+        J.gsc0 = s.gsc0; J.gsc1 = s.gsc1;
+        J.accumulate = s.accumulate;
+        J.adj = s.adj;
+        const int ntile = (C.nh + 3) / 4, ng4 = (J.ngf + 3) / 4;
+        if (synth) {
+            J.ltask0 = B.ltask;
+            J.ltasks = ng4 * ntile;
+            B.ltask += J.ltasks;
+            // staging band: every coefficient when that fits beside the FFT buffers' LDS,
+            // else bands of whole orders of at most chl coefficients (chl >= T+1)
+            const int fft = aer::fz_fftbytes(h->nlon, csz);
+            const int per = 4 * 2 * csz + 2 * 4 * csz;   // 4 complex coefficients + 2 table rows of 4
+            const int budget = std::max(fft, h->band_kb * 1024);
+            J.chl = std::max(C.T + 1, std::min(C.nlm, budget / per));
+            // segment stride = 8 mod 32 coefficients: the four segments' operand reads of
+            // one MFMA step then start 16 banks apart
+            J.chl = (J.chl + 23) / 32 * 32 + 8;
+            B.lds = std::max(B.lds, (size_t)(aer::fz_fbytes(C.T, csz) + std::max(fft, aer::fz_stage_bytes(J.chl, csz))));
+        } else {
+            J.ftask0 = B.ftask;
+            B.ftask += ng4 * ntile;
+            J.four_off = (long)B.four_bytes;
+            B.four_bytes += ((size_t)aer::fz_four_elems(J.ngf, C.T, C.nh) * csz + 255) / 256 * 256;
+            const bool scalar = s.kind == aer::ANA_SCALAR;
+            const int ncg = scalar ? (s.nitem + 3) / 4 : (s.nitem + 1) / 2;
+            J.ltask0 = B.ltask;
+            J.ltasks = ncg * ((C.T + 2) / 2);
+            B.ltask += J.ltasks;
+            B.lds = std::max(B.lds, (size_t)aer::fz_fftbytes(h->nlon, csz));
+            const int nlgf = scalar ? 4 : 2 * (s.gin2 ? 3 : 2);
+            B.lds_leg = std::max(B.lds_leg, (size_t)aer::la_lds(nlgf, C.nh, csz));
+        }
+        B.kmax = std::max(B.kmax, C.T + 1);
+        B.jobs.push_back(J);
+    }
+    if (B.lds > 160 * 1024 || B.lds_leg > 160 * 1024) return false;
+    B.pm = (any64 && any32) ? aer::PM_MIXED : (any32 ? aer::PM_FP32 : aer::PM_FP64);
+    return true;
+}
+
+// Grid of a tiled launch: whole rounds of the 8 XCDs (aer::xcd_task).
+int xcd_grid(int ntask) { return (ntask + 7) / 8 * 8; }
+
+template <int PM, bool PLAN>
+int launch_tiled_pm(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
+                    const int *tj_leg, const int *tj_fft)
+{
+    aer::LaunchArgs LL = L, LF = L;
+    LL.task_job = tj_leg;
+    LF.task_job = tj_fft;
+    LL.kmax = LF.kmax = B.kmax;
+    LL.ntask = B.ltask;
+    LF.ntask = B.ftask;
+    if (synth) {
+        return timed_launch(th, 4, [&] {
+            hipLaunchKernelGGL((aer::k_syn_fused<PM, PLAN>), dim3(xcd_grid(B.ltask)), dim3(aer::FZ_THREADS), B.lds, st, LL);
+        });
+    }
+    int rc = timed_launch(th, 2, [&] {
+        hipLaunchKernelGGL((aer::k_fft_anal4<PM, PLAN>), dim3(xcd_grid(B.ftask)), dim3(aer::FZ_THREADS), B.lds, st, LF);
+    });
+    if (rc) return rc;
+    return timed_launch(th, 3, [&] {
+        hipLaunchKernelGGL((aer::k_leg_anal4<PM, PLAN>), dim3(xcd_grid(B.ltask)), dim3(aer::LA_THREADS), B.lds_leg, st, LL);
+    });
+}
+
 template <int PM, bool PLAN>
 int launch_table_pm(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Table &B, bool synth,
                      const int *tj_leg, const int *tj_fft)
@@ -506,6 +681,19 @@ int launch_table(aer_sht *th, hipStream_t st, const aer::LaunchArgs &L, const Ta
                  const int *tj_leg = nullptr, const int *tj_fft = nullptr)
 {
     const bool plan = L.djobs != nullptr;
+    if (B.tiled) {
+        switch (B.pm) {
+            case aer::PM_FP64:
+                return plan ? launch_tiled_pm<aer::PM_FP64, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                            : launch_tiled_pm<aer::PM_FP64, false>(th, st, L, B, synth, tj_leg, tj_fft);
+            case aer::PM_FP32:
+                return plan ? launch_tiled_pm<aer::PM_FP32, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                            : launch_tiled_pm<aer::PM_FP32, false>(th, st, L, B, synth, tj_leg, tj_fft);
+            default:
+                return plan ? launch_tiled_pm<aer::PM_MIXED, true>(th, st, L, B, synth, tj_leg, tj_fft)
+                            : launch_tiled_pm<aer::PM_MIXED, false>(th, st, L, B, synth, tj_leg, tj_fft);
+        }
+    }
     switch (B.pm) {
         case aer::PM_FP64:
             return plan ? launch_table_pm<aer::PM_FP64, true>(th, st, L, B, synth, tj_leg, tj_fft)
@@ -544,7 +732,7 @@ int run_jobs(aer_sht *h, JobSpec *specs, int njob, bool synth)
         if (rc) return rc;
     }
     Table B;
-    build_table(hs, specs, njob, synth, B);
+    if (!build_table_tiled(hs, specs, njob, synth, B)) build_table(hs, specs, njob, synth, B);
     if (B.jobs.empty()) return AER_OK;
     int rc = h->four.ensure(B.four_bytes);
     if (rc) return rc;
@@ -695,13 +883,20 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
     legendre_tables(h->lmax, h->nh, h->nhp, h->ct, P, D, Q);
     const std::vector<double> *tabs[3] = {&P, &D, &Q};
     h->kp = (h->lmax / 2 + 1 + 15) / 16 * 16;
+    h->nq = ((h->lmax + 2) / 2 + 3) / 4;
     for (int k = 0; k < 3; ++k) {
         if ((rc = upload(h->tab64[k], tabs[k]->data(), tabs[k]->size() * sizeof(double)))) return rc;
         const std::vector<double> tt = transpose_table(*tabs[k], h->lmax, h->nhp, h->kp);
         if ((rc = upload(h->tabT64[k], tt.data(), tt.size() * sizeof(double)))) return rc;
+        const std::vector<double> tz = tile_table(*tabs[k], h->nlm, h->nhp, (h->nh + 3) / 4);
+        if ((rc = upload(h->tabZ64[k], tz.data(), tz.size() * sizeof(double)))) return rc;
+        const std::vector<double> ty = quad_table(*tabs[k], h->lmax, h->nhp, (h->nh + 3) / 4 * 4, h->nq);
+        if ((rc = upload(h->tabY64[k], ty.data(), ty.size() * sizeof(double)))) return rc;
     }
     for (auto &b : h->tab32) b.release();
     for (auto &b : h->tabT32) b.release();
+    for (auto &b : h->tabZ32) b.release();
+    for (auto &b : h->tabY32) b.release();
     // per-degree factors
     const int T = h->lmax;
     std::vector<double> lf((size_t)AER_NMODE * (T + 1), 0.0);
@@ -717,6 +912,13 @@ int aer_sht_set_grid(aer_sht_t h, int layout, double eps, int nlat, int nphi)
         lf[aer::M_ZERO * (T + 1) + l] = 0.0;
     }
     if ((rc = upload(h->lfac, lf.data(), lf.size() * sizeof(double)))) return rc;
+    {
+        std::vector<double> lfm((size_t)AER_NMODE * h->nlm);
+        for (int md = 0; md < AER_NMODE; ++md)
+            for (int m = 0, lm = 0; m <= T; ++m)
+                for (int l = m; l <= T; ++l, ++lm) lfm[(size_t)md * h->nlm + lm] = lf[md * (T + 1) + l];
+        if ((rc = upload(h->lfacm, lfm.data(), lfm.size() * sizeof(double)))) return rc;
+    }
     std::vector<double> wp(h->nhp, 0.0);
     for (int j = 0; j < h->nh; ++j) {
         wp[j] = h->w[j] * 2.0 * kPi / nphi;
@@ -860,7 +1062,12 @@ int aer_sht_destroy(aer_sht_t h)
     for (auto &b : h->tab32) b.release();
     for (auto &b : h->tabT64) b.release();
     for (auto &b : h->tabT32) b.release();
+    for (auto &b : h->tabZ64) b.release();
+    for (auto &b : h->tabZ32) b.release();
+    for (auto &b : h->tabY64) b.release();
+    for (auto &b : h->tabY32) b.release();
     h->lfac.release();
+    h->lfacm.release();
     h->wpair.release();
     h->tw64.release();
     h->tw32.release();
@@ -888,6 +1095,14 @@ int aer_sht_set_precision(aer_sht_t h, int prec)
     if (prec != AER_PREC_FP64 && prec != AER_PREC_FP32 && prec != AER_PREC_SPLIT)
         return fail(AER_EINVAL, "aer_sht_set_precision: unsupported precision");
     h->prec = prec;
+    return AER_OK;
+}
+
+int aer_sht_set_path(aer_sht_t h, int path)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (path != AER_PATH_TILED && path != AER_PATH_STAGED) return fail(AER_EINVAL, "aer_sht_set_path: unknown path");
+    h->path_syn = h->path_ana = path;
     return AER_OK;
 }
 
@@ -950,21 +1165,20 @@ struct aer_plan {
     Table syn, ana;
     DevBuf djob_syn, djob_ana, four;
     DevBuf tj[4];                    // task -> job: synthesis Legendre, FFT; analysis FFT, Legendre
+    std::vector<aer_sht *> hsy, han; // the jobs of each direction and their handles (for rebuilds)
+    std::vector<JobSpec> ssy, san;
 };
 
 namespace {
 
-int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &specs)
+int plan_build(aer_plan *p)
 {
-    std::vector<aer_sht *> hsy, han;
-    std::vector<JobSpec> ssy, san;
-    for (size_t k = 0; k < specs.size(); ++k) {
-        const bool synth = specs[k].kind <= aer::SYN_GRAD;
-        (synth ? hsy : han).push_back(hs[k]);
-        (synth ? ssy : san).push_back(specs[k]);
-    }
-    build_table(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
-    build_table(han.data(), san.data(), (int)san.size(), false, p->ana);
+    std::vector<aer_sht *> &hsy = p->hsy, &han = p->han;
+    std::vector<JobSpec> &ssy = p->ssy, &san = p->san;
+    if (!build_table_tiled(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn))
+        build_table(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
+    if (!build_table_tiled(han.data(), san.data(), (int)san.size(), false, p->ana))
+        build_table(han.data(), san.data(), (int)san.size(), false, p->ana);
     int rc;
     if (!p->syn.jobs.empty() && (rc = upload(p->djob_syn, p->syn.jobs.data(), p->syn.jobs.size() * sizeof(aer::Job))))
         return rc;
@@ -975,7 +1189,7 @@ int plan_build(aer_plan *p, std::vector<aer_sht *> &hs, std::vector<JobSpec> &sp
     for (int d = 0; d < 2; ++d) {
         const Table &B = *tabs[d];
         if (B.jobs.empty()) continue;
-        std::vector<int> tl(B.ltask), tf(B.ftask);
+        std::vector<int> tl(std::max(B.ltask, 1), 0), tf(std::max(B.ftask, 1), 0);
         for (size_t k = 0; k < B.jobs.size(); ++k) {
             const int l1 = k + 1 < B.jobs.size() ? B.jobs[k + 1].ltask0 : B.ltask;
             const int f1 = k + 1 < B.jobs.size() ? B.jobs[k + 1].ftask0 : B.ftask;
@@ -1031,7 +1245,12 @@ int aer_plan_create(int njob, const aer_sht_t *cfgs, const aer_job *jobs, aer_pl
     }
     aer_plan *p = new aer_plan();
     p->th = hs[0];
-    rc = plan_build(p, hs, specs);
+    for (int k = 0; k < njob; ++k) {
+        const bool synth = specs[k].kind <= aer::SYN_GRAD;
+        (synth ? p->hsy : p->han).push_back(hs[k]);
+        (synth ? p->ssy : p->san).push_back(specs[k]);
+    }
+    rc = plan_build(p);
     if (rc) {
         aer_plan_destroy(p);
         return rc;
@@ -1055,6 +1274,114 @@ int aer_plan_destroy(aer_plan_t p)
     for (auto &b : p->tj) b.release();
     delete p;
     return AER_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Autotuner candidates: the staged path, and the tiled path with m-band staging budgets
+// of 16, 32 and 64 KiB (bands of whole orders; smaller bands leave more LDS for
+// co-resident blocks, larger ones need fewer barriers).
+struct TuneCand {
+    int path, band_kb;
+};
+constexpr TuneCand kTune[AER_TUNE_NCAND] = {{AER_PATH_STAGED, 64}, {AER_PATH_TILED, 16}, {AER_PATH_TILED, 32},
+                                            {AER_PATH_TILED, 64}};
+
+void tune_apply(aer_sht *h, bool synth, const TuneCand &c)
+{
+    (synth ? h->path_syn : h->path_ana) = c.path;
+    if (synth) h->band_kb = c.band_kb;
+}
+
+// Mean milliseconds of `run` over reps launches on stream st after one warm-up.
+template <class F>
+int tune_time(hipStream_t st, int reps, F &&run, double &ms)
+{
+    int rc = run();
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, st));
+    for (int r = 0; r < reps && !rc; ++r) rc = run();
+    HIP_TRY(hipEventRecord(b, st));
+    HIP_TRY(hipEventSynchronize(b));
+    float f = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&f, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ms = (double)f / std::max(reps, 1);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_autotune_jobs(aer_sht_t h, int njob, const aer_job *jobs, int reps, double *times_ms, int *choice)
+{
+    if (!h) return fail(AER_EINVAL, "null handle");
+    if (njob <= 0 || njob > AER_MAX_JOBS || !jobs) return fail(AER_EINVAL, "aer_autotune_jobs: bad job count");
+    if (reps <= 0) return fail(AER_EINVAL, "aer_autotune_jobs: reps must be positive");
+    JobSpec spec[AER_MAX_JOBS];
+    const bool synth = jobs[0].kind <= AER_JOB_SYNTH_GRAD;
+    for (int k = 0; k < njob; ++k) {
+        int rc = job_to_spec(h, jobs[k], spec[k]);
+        if (rc) return rc;
+        if ((jobs[k].kind <= AER_JOB_SYNTH_GRAD) != synth)
+            return fail(AER_EINVAL, "aer_autotune_jobs: synthesis and analysis jobs cannot share a launch");
+    }
+    int best = 0;
+    double tbest = 0.0;
+    for (int c = 0; c < AER_TUNE_NCAND; ++c) {
+        tune_apply(h, synth, kTune[c]);
+        double ms = 0.0;
+        int rc = tune_time(h->stream, reps, [&] { return run_jobs(h, spec, njob, synth); }, ms);
+        if (rc) return rc;
+        if (times_ms) times_ms[c] = ms;
+        if (c == 0 || ms < tbest) { best = c; tbest = ms; }
+    }
+    tune_apply(h, synth, kTune[best]);
+    if (choice) *choice = best;
+    return AER_OK;
+}
+
+int aer_plan_autotune(aer_plan_t p, void *stream, int reps, double *times_ms, int *choice)
+{
+    if (!p) return fail(AER_EINVAL, "aer_plan_autotune: null plan");
+    if (reps <= 0) return fail(AER_EINVAL, "aer_plan_autotune: reps must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    for (int d = 0; d < 2; ++d) {
+        const bool synth = d == 0;
+        std::vector<aer_sht *> &hs = synth ? p->hsy : p->han;
+        if (hs.empty()) {
+            if (choice) choice[d] = 0;
+            continue;
+        }
+        int best = 0;
+        double tbest = 0.0;
+        for (int c = 0; c < AER_TUNE_NCAND; ++c) {
+            for (aer_sht *h : hs) tune_apply(h, synth, kTune[c]);
+            int rc = plan_build(p);
+            if (rc) return rc;
+            aer::LaunchArgs L{};
+            L.four = p->four.p;
+            const Table &B = synth ? p->syn : p->ana;
+            L.njob = (int)B.jobs.size();
+            L.djobs = (const aer::Job *)(synth ? p->djob_syn.p : p->djob_ana.p);
+            const int *tl = (const int *)p->tj[synth ? 0 : 3].p, *tf = (const int *)p->tj[synth ? 1 : 2].p;
+            double ms = 0.0;
+            rc = tune_time(st, reps, [&] { return launch_table(p->th, st, L, B, synth, tl, tf); }, ms);
+            if (rc) return rc;
+            if (times_ms) times_ms[d * AER_TUNE_NCAND + c] = ms;
+            if (c == 0 || ms < tbest) { best = c; tbest = ms; }
+        }
+        for (aer_sht *h : hs) tune_apply(h, synth, kTune[best]);
+        if (choice) choice[d] = best;
+    }
+    return plan_build(p);
 }
 
 int aer_run_jobs_d(aer_sht_t h, int njob, const aer_job *jobs)

@@ -51,9 +51,14 @@ struct CfgDev {
     const void *tabP, *tabD, *tabQ;        // [nlm][nhp] in the compute type
     const void *tabPT, *tabDT, *tabQT;     // [m][parity][nhp][kp]: degree l = m + parity + 2k fastest
     int kp;                                // k extent of the transposed tables (multiple of 16)
+    const void *tabZP, *tabZD, *tabZQ;     // tiled synthesis: [tile of 4 pairs][lm][4]
+    const void *tabYP, *tabYD, *tabYQ;     // tiled analysis: FP64 [m][parity][row quad][pair (nh rounded to 4)][4],
+                                           // FP32 [m][parity][4 row quads][pair][16]
+    int nq;                                // row quads of the tiled analysis tables: ((T+2)/2 + 3)/4
     const int *expP, *expD, *expQ;         // split form: row exponents [m][parity][nhp]
     const int *expPT, *expDT, *expQT;      // split form: row exponents [m][parity][kp]
     const double *lfac;                    // [AER_NMODE][T+1]
+    const double *lfacm;                   // [AER_NMODE][nlm]: the same factors per coefficient
     const double *wpair;                   // [nhp] w_j * 2pi/nlon (x1/2 on a self-mirrored equator row)
     const void *tw;                        // per-pass forward twiddles (generic FFT path)
     const void *twN;                       // e^{-2 pi i k/nlon}, k < nlon (compile-time FFT paths)
@@ -79,6 +84,7 @@ struct Job {
                                       // m > 0 and weight rows by w_j 2pi/nlon; analysis jobs use
                                       // unit row weights and double m > 0
     int ltasks;                       // Legendre tasks of the job (its range is padded to whole blocks)
+    int chl;                          // tiled synthesis: coefficients per staging band
     long four_off;                    // byte offset of the job's intermediate region
 };
 
@@ -788,7 +794,7 @@ __host__ __device__ constexpr int fft_fixed_rows(int n) { return n == 64 ? 16 : 
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 template <int ROWS, int N> constexpr int padded_len() { return ROWS * N + (ROWS * N) / 16 + 1; }
 
-template <int S, class T, int N, int ROWS, int P>
+template <int S, class T, int N, int ROWS, int P, int NT = AER_FFT_THREADS>
 __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__restrict__ dst,
                                       const cx<T> *__restrict__ tw)
 {
@@ -798,7 +804,6 @@ __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__re
     constexpr int NR = N / R;
     constexpr int TS = N / (Ns * R);
     constexpr int TOT = ROWS * NR;
-    constexpr int NT = AER_FFT_THREADS;
 #pragma unroll
     for (int it = 0; it < (TOT + NT - 1) / NT; ++it) {
         const int bi = threadIdx.x + it * NT;
@@ -824,16 +829,16 @@ __device__ inline void fft_pass_fixed(const cx<T> *__restrict__ src, cx<T> *__re
     }
 }
 
-template <int S, class T, int N, int ROWS, int P = 0>
+template <int S, class T, int N, int ROWS, int NT = AER_FFT_THREADS, int P = 0>
 __device__ inline cx<T> *fft_block_fixed(cx<T> *a, cx<T> *b, const cx<T> *tw)
 {
     constexpr RadixPlan PL = radix_plan(N);
     if constexpr (P >= PL.npass) {
         return a;
     } else {
-        fft_pass_fixed<S, T, N, ROWS, P>(a, b, tw);
+        fft_pass_fixed<S, T, N, ROWS, P, NT>(a, b, tw);
         __syncthreads();
-        return fft_block_fixed<S, T, N, ROWS, P + 1>(b, a, tw);
+        return fft_block_fixed<S, T, N, ROWS, NT, P + 1>(b, a, tw);
     }
 }
 

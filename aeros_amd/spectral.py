@@ -92,6 +92,11 @@ class AerosSht:
         self.precision = precision
         check(lib().aer_sht_set_precision(self._h, PRECISIONS[precision]))
 
+    def set_path(self, path):
+        """Kernel path: "tiled" (fused synthesis; default) or "staged" (HBM intermediate)."""
+        self.path = path
+        check(lib().aer_sht_set_path(self._h, {"tiled": _lib.AER_PATH_TILED, "staged": _lib.AER_PATH_STAGED}[path]))
+
     def _bind_stream(self):
         import torch
         check(lib().aer_sht_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))

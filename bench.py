@@ -35,7 +35,12 @@ sys.path.insert(0, ROOT)
 T, NLEV = 42, 20
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
-KIND_NAMES = ["k_leg_synth", "k_fft_synth", "k_fft_anal", "k_leg_anal"]
+# kernel kinds of aer_sht_timing: 0-3 the staged path, 2-4 the tiled path
+NKIND = 5
+KIND_NAMES = {"staged": ["k_leg_synth", "k_fft_synth", "k_fft_anal", "k_leg_anal", None],
+              "tiled": [None, None, "k_fft_anal4", "k_leg_anal4", "k_syn_fused"]}
+# autotuner candidates (aer_autotune_jobs)
+TUNE_NAMES = ["staged", "tiled/16KiB", "tiled/32KiB", "tiled/64KiB"]
 
 
 def set_counts(L):
@@ -45,7 +50,9 @@ def set_counts(L):
 
 
 def algorithmic(nlon, nlat, nlm, L):
-    """Algorithmic bytes (FP64 I/O) and dense flops per set, per kernel kind.
+    """Algorithmic bytes (FP64 I/O) and dense flops per set, per kernel kind
+    (0 Legendre synthesis, 1 FFT synthesis, 2 FFT analysis, 3 Legendre analysis,
+    4 the fused synthesis = 0 + 1).
 
     G = 8 nlon nlat, S = 16 nlm; Legendre flops per scalar contraction 4 nlat nlm;
     a vector transform is 4 contractions, a gradient 2; FFT 2.5 nlon log2(nlon) per row."""
@@ -61,6 +68,8 @@ def algorithmic(nlon, nlat, nlm, L):
              F1 * (c["syn_scalar"] + 2 * c["syn_vector"] + 2 * c["syn_grad"]),
              F1 * (c["ana_scalar"] + 2 * c["ana_vector"]),
              L1 * (c["ana_scalar"] + 4 * c["ana_vector"])]
+    bytes_.append(bytes_[0] + bytes_[1])
+    flops.append(flops[0] + flops[1])
     return bytes_, flops
 
 
@@ -137,6 +146,22 @@ class TransformSet:
         js, ns, ja, na = self._jl
         check(lib().aer_run_jobs_d(self.sht._h, ns, js))
         check(lib().aer_run_jobs_d(self.sht._h, na, ja))
+
+    def autotune(self, reps=20):
+        """Kernel configuration per direction by measurement (aer_autotune_jobs): the staged
+        path, or the tiled path with an m-band staging budget of 16, 32 or 64 KiB."""
+        from aeros_amd._lib import AER_TUNE_NCAND, check, lib
+        if getattr(self, "_jl", None) is None:
+            self._jl = self._jobs()
+        js, ns, ja, na = self._jl
+        out = {}
+        for name, (jl, n) in (("synthesis", (js, ns)), ("analysis", (ja, na))):
+            t = (ctypes.c_double * AER_TUNE_NCAND)()
+            c = ctypes.c_int()
+            check(lib().aer_autotune_jobs(self.sht._h, n, jl, reps, t, ctypes.byref(c)))
+            out[name] = {"ms": [round(t[k], 5) for k in range(AER_TUNE_NCAND)], "kept": TUNE_NAMES[c.value]}
+        self.tuned = out
+        return out
 
     def run_calls(self):
         """The same set through the per-purpose entry points (4 calls, 8 kernels)."""
@@ -221,11 +246,11 @@ def kernel_times(sht, fn, steps):
     check(lib().aer_sht_timing(sht._h, 1))
     for _ in range(steps):
         fn()
-    ms = (ctypes.c_double * 4)()
-    n = (ctypes.c_int * 4)()
+    ms = (ctypes.c_double * NKIND)()
+    n = (ctypes.c_int * NKIND)()
     check(lib().aer_sht_timing_read(sht._h, ms, n))
     check(lib().aer_sht_timing(sht._h, 0))
-    return [ms[k] / steps for k in range(4)], [n[k] / steps for k in range(4)]
+    return [ms[k] / steps for k in range(NKIND)], [n[k] / steps for k in range(NKIND)]
 
 
 def rel_err(a, ref):
@@ -243,7 +268,7 @@ def sref_bytes(nlon, nlat, nlm, L):
     return Ns * (G + S) + 2 * Nv * (G + S) + Ng * (S + 2 * G)
 
 
-def ensemble_config2(torch, precision, steps=50, warmup=5):
+def ensemble_config2(torch, precision, steps=50, warmup=5, path="auto"):
     """64 Held-Suarez members of uneven size (24 x T21L16, 24 x T31L20, 16 x T42L20): the
     S_ref transform set of every member in one compiled plan (4 launches), replayed from a
     HIP graph. Returns throughput and the HBM rate of the algorithmic traffic."""
@@ -252,6 +277,8 @@ def ensemble_config2(torch, precision, steps=50, warmup=5):
     from aeros_amd.spectral import AerosSht
     mix = [(21, 16, 24), (31, 20, 24), (42, 20, 16)]
     shts = {T: AerosSht(T, precision=precision) for T, _, _ in mix}
+    for s_ in shts.values():
+        s_.set_path("staged" if path == "auto" else path)
     cfgs, jobs, keep, nbytes = [], [], [], 0
     a = 1.0 / 6.371e6
     P = lambda t: t.data_ptr()
@@ -294,7 +321,15 @@ def ensemble_config2(torch, precision, steps=50, warmup=5):
     n = len(jobs)
     plan = ctypes.c_void_p()
     L.check(L.lib().aer_plan_create(n, (ctypes.c_void_p * n)(*cfgs), (L.AerJob * n)(*jobs), ctypes.byref(plan)))
-    run = lambda: L.check(L.lib().aer_plan_run(plan, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    stream = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    tuned = None
+    if path == "auto":
+        t = (ctypes.c_double * (2 * L.AER_TUNE_NCAND))()
+        c = (ctypes.c_int * 2)()
+        L.check(L.lib().aer_plan_autotune(plan, stream(), 10, t, c))
+        tuned = {d: {"ms": [round(t[i * L.AER_TUNE_NCAND + k], 5) for k in range(L.AER_TUNE_NCAND)],
+                     "kept": TUNE_NAMES[c[i]]} for i, d in enumerate(("synthesis", "analysis"))}
+    run = lambda: L.check(L.lib().aer_plan_run(plan, stream()))
     run()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
@@ -305,8 +340,8 @@ def ensemble_config2(torch, precision, steps=50, warmup=5):
     L.lib().aer_plan_destroy(plan)
     ntr = sum(count * (11 * nlev + 4) for _, nlev, count in mix)
     return {"workload": "64 members: 24 x T21L16 + 24 x T31L20 + 16 x T42L20, S_ref set (11L+4 transforms) each, "
-                        "one compiled plan (4 launches) replayed from a HIP graph",
-            "precision": precision, "jobs": n, "ms_per_step": round(ms, 4),
+                        "one compiled plan (3-4 launches) replayed from a HIP graph",
+            "precision": precision, "jobs": n, "ms_per_step": round(ms, 4), "autotune": tuned,
             "ensemble_steps_per_s": round(1000.0 / ms, 1),
             "scalar_eq_transforms_per_s": round(ntr * 1000.0 / ms, 0),
             "alg_bytes_per_step": int(nbytes),
@@ -314,7 +349,7 @@ def ensemble_config2(torch, precision, steps=50, warmup=5):
             "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
-def dynamics_step(torch, precision, steps=20, warmup=3):
+def dynamics_step(torch, precision, steps=20, warmup=3, path="staged"):
     """Whole Held-Suarez dynamics steps at T42L20 (transform halves, column terms +
     forcing, semi-implicit leapfrog with diffusion and filter, mass fixer), eager
     launches, all state on the device. Returns ms per step."""
@@ -324,6 +359,7 @@ def dynamics_step(torch, precision, steps=20, warmup=3):
     A, B = vgrid_init(NLEV, stretch_a=1.0, sigma_t=0.0, p_top=0.0)
     st = Stepper(T, A, B, 1200.0, tau_diff=0.1 * 86400.0, eps_filter=0.06, raw_alpha=0.53,
                  held_suarez=True, mass_fixer=True, precision=precision)
+    st.sht.set_path("staged" if path == "auto" else path)
     nlm = st.sht.nlm
     vor = red_spectra(T, NLEV, 3, field0=0, zero_l0=True) * 1e-6
     temp = np.zeros((NLEV, nlm), complex)
@@ -367,6 +403,9 @@ def main():
     ap.add_argument("--only", choices=["set", "ensemble", "step"], default=None,
                     help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms), only the "
                          "ensemble, or only whole Held-Suarez dynamics steps")
+    ap.add_argument("--path", choices=["auto", "tiled", "staged"], default="auto",
+                    help="kernel path of the transforms: the autotuner's choice per direction (default), "
+                         "or one path for both (aer_sht_set_path)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -387,21 +426,27 @@ def main():
         print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 8, "launches": 5}))
         return
     if args.only == "step":
-        print(json.dumps({"held_suarez_step_ms": dynamics_step(torch, args.precision, args.steps, args.warmup),
-                          "precision": args.precision}))
+        print(json.dumps({"held_suarez_step_ms": dynamics_step(torch, args.precision, args.steps, args.warmup,
+                                                               args.path),
+                          "precision": args.precision, "kernel_path": args.path}))
         return
     if args.only == "ensemble":
-        print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup)}))
+        print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup,
+                                                               args.path)}))
         return
     sht = AerosSht(T)
+    sht.set_path("staged" if args.path == "auto" else args.path)
     ts = TransformSet(sht, torch)
+    tune = (lambda: ts.autotune()) if args.path == "auto" else (lambda: None)
     if args.only == "set":
         from aeros_amd import _lib as L
         sht.set_precision(args.precision)
         if args.precision == "fp32":
             ts.set_assembly_prec(L.AER_JOB_PREC_FP64)
+        tuned = tune()
         ms = time_steps(torch, ts.run_batched, args.steps, args.warmup)
-        print(json.dumps({"set_ms_per_step": ms, "precision": args.precision}))
+        print(json.dumps({"set_ms_per_step": ms, "precision": args.precision, "kernel_path": args.path,
+                          "autotune": tuned}))
         return
     sht._bind_stream()
 
@@ -409,6 +454,7 @@ def main():
     sht.set_precision("fp64")
     ts.run_batched()
     ref = ts.outputs()
+    tuned64 = tune()
     plain_steps = max(5, args.steps // 10)
     # the per-level tendency call also re-analyses dlnps/dt on every level: 20 extra
     # single-level transforms beyond the set's 264; normalize the plain time to 264
@@ -429,6 +475,7 @@ def main():
         # ... and the measured form: the cancellation-prone (zeta_t, D_t - lap(Phi+K))
         # analysis job in FP64 inside the same fused launch
         ts.set_assembly_prec(L.AER_JOB_PREC_FP64)
+    tuned = tune()
     ts.run_batched()
     errs = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
     ms_eager = time_steps(torch, ts.run_batched, args.steps, args.warmup)
@@ -455,22 +502,30 @@ def main():
     bytes_k, flops_k = algorithmic(sht.nlon, sht.nlat, sht.nlm, NLEV)
     set_bytes = sum(bytes_k)
     kernels = {}
-    for k in range(4):
+    if args.path == "auto":
+        sp = "staged" if tuned["synthesis"]["kept"] == "staged" else "tiled"
+        ap_ = "staged" if tuned["analysis"]["kept"] == "staged" else "tiled"
+        names = KIND_NAMES[sp][:2] + KIND_NAMES[ap_][2:4] + KIND_NAMES[sp][4:]
+    else:
+        names = KIND_NAMES[args.path]
+    for k in range(NKIND):
+        if not klaunch[k]:
+            continue
         hbm = bytes_k[k] / (kms[k] * 1e-3) / 1e9
         tf = flops_k[k] / (kms[k] * 1e-3) / 1e12
-        kernels[KIND_NAMES[k]] = {"ms_per_set": round(kms[k], 5), "launches_per_set": klaunch[k],
-                                  "alg_bytes_per_set": int(bytes_k[k]), "alg_flops_per_set": int(flops_k[k]),
-                                  "hbm_GBs": round(hbm, 1), "TFLOPs": round(tf, 3),
-                                  "fp64_ms_per_set": round(k64_ms[k], 5)}
+        kernels[names[k]] = {"ms_per_set": round(kms[k], 5), "launches_per_set": klaunch[k],
+                             "alg_bytes_per_set": int(bytes_k[k]), "alg_flops_per_set": int(flops_k[k]),
+                             "hbm_GBs": round(hbm, 1), "TFLOPs": round(tf, 3),
+                             "fp64_ms_per_set": round(k64_ms[k], 5)}
     dom = int(np.argmax(kms))
     pk = PEAK_TFLOPS[args.precision]
     if dom in (0, 3):   # Legendre GEMM: MFMA-bound by its dense flop count
         ach = flops_k[dom] / (kms[dom] * 1e-3) / 1e12
-        roof = {"kernel": KIND_NAMES[dom], "bound": "mfma", "achieved": round(ach, 3), "peak": pk,
+        roof = {"kernel": names[dom], "bound": "mfma", "achieved": round(ach, 3), "peak": pk,
                 "unit": "TFLOP/s", "frac": round(ach / pk, 5), "traffic": None}
-    else:
+    else:   # FFT stages and the fused synthesis: HBM-bound by their algorithmic bytes
         ach = bytes_k[dom] / (kms[dom] * 1e-3) / 1e9
-        roof = {"kernel": KIND_NAMES[dom], "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+        roof = {"kernel": names[dom], "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None}
     roof["alg_per_launch"] = "see kernels[].alg_*_per_set / launches_per_set"
 
@@ -490,7 +545,10 @@ def main():
                    "precision": ("fp32-mfma; the (zeta_t, D_t) assembly job fp64 in the same launch"
                                  if args.precision == "fp32" else "fp64-mfma"),
                    "io": "fp64 in HBM"},
-        "timing": "HIP graph replay of one set per step (4 kernels: fused synthesis and analysis launch pairs)",
+        "timing": "HIP graph replay of one set per step (the kernels of the configuration below)",
+        "kernel_path": args.path,
+        "autotune": tuned,
+        "autotune_fp64": tuned64,
         "eager_sets_per_s": round(1000.0 / ms_eager, 2),
         "eager_4_calls_sets_per_s": round(1000.0 / ms_calls, 2),
         "fp64_batched_sets_per_s": round(1000.0 / ms_fp64, 2),
@@ -509,12 +567,12 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
-    line["ensemble_config2"] = ensemble_config2(torch, args.precision)
+    line["ensemble_config2"] = ensemble_config2(torch, args.precision, path=args.path)
     line["held_suarez_step_T42L20"] = {
         "what": "one whole dynamics step: transform halves, column terms + HS forcing, semi-implicit "
                 "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
-        "ms_per_step_fp64": dynamics_step(torch, "fp64"),
-        "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32")}
+        "ms_per_step_fp64": dynamics_step(torch, "fp64", path=args.path),
+        "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -110,6 +110,16 @@ int aer_sht_destroy(aer_sht_t cfg);
 int aer_sht_set_precision(aer_sht_t cfg, int prec);
 /* HIP stream (hipStream_t) for subsequent device-pointer calls; NULL = default stream. */
 int aer_sht_set_stream(aer_sht_t cfg, void *stream);
+/* Kernel path of subsequent transforms on this handle, both directions (no reference
+ * counterpart; SHTns picks its own kernels). AER_PATH_STAGED (default): Legendre and FFT
+ * as separate launches with an HBM intermediate, every precision and grid length.
+ * AER_PATH_TILED: one fused Legendre+FFT synthesis launch (4 grid fields x 4 latitude
+ * pairs per block, 4x4 MFMA) and a two-launch analysis, for FP64 and FP32 jobs on the
+ * compile-time grid lengths 64, 96, 128, 160, 192, 256 and 320; other jobs fall back to
+ * the staged path. Both give the same results to rounding. */
+#define AER_PATH_TILED  0
+#define AER_PATH_STAGED 1
+int aer_sht_set_path(aer_sht_t cfg, int path);
 
 /* ---- per-call transforms on HOST arrays, one field; synchronous.
  * Drop-in replacements for the five SHTns calls aeros makes. Analyses leave
@@ -339,13 +349,25 @@ int aer_memcpy_d2h(void *dst, const void *src, size_t bytes);
 /* Block until all work queued on the handle's stream has finished. */
 int aer_sht_sync(aer_sht_t cfg);
 
+/* ---- autotuner (no reference counterpart). Times a job list (one direction, as for
+ * aer_run_jobs_d) or a compiled plan under every kernel configuration and keeps the
+ * fastest for that direction on the handles involved. Candidates: 0 the staged path;
+ * 1, 2, 3 the tiled path with m-band staging budgets of 16, 32 and 64 KiB. Each candidate
+ * runs once untimed and reps times timed; synchronous; the jobs' outputs are overwritten.
+ * times_ms (optional) receives the mean milliseconds per candidate (a plan: synthesis
+ * candidates, then analysis candidates), choice (optional) the kept index (a plan: two). */
+#define AER_TUNE_NCAND 4
+int aer_autotune_jobs(aer_sht_t cfg, int njob, const aer_job *jobs, int reps, double *times_ms, int *choice);
+int aer_plan_autotune(aer_plan_t plan, void *stream, int reps, double *times_ms, int *choice);
+
 /* ---- observability: per-kernel timing with HIP events recorded on the handle's
  * stream around every kernel launch (off by default; no reference counterpart --
  * the reference times whole transforms with system_clock, drivers/bench_sht.f90:214-227).
- * Kernel kinds: 0 Legendre synthesis, 1 FFT synthesis, 2 FFT analysis, 3 Legendre analysis.
+ * Kernel kinds: 0 Legendre synthesis, 1 FFT synthesis, 2 FFT analysis, 3 Legendre analysis,
+ * 4 fused Legendre+FFT synthesis (the tiled path).
  * aer_sht_timing(cfg, 1) starts and clears; aer_sht_timing_read() synchronizes the stream
  * and returns the summed milliseconds and launch counts per kind since the start. */
-#define AER_NKERNEL_KIND 4
+#define AER_NKERNEL_KIND 5
 int aer_sht_timing(aer_sht_t cfg, int enable);
 int aer_sht_timing_read(aer_sht_t cfg, double *total_ms, int *launches);
 

@@ -110,6 +110,11 @@ SIGNATURES["aer_semiimp_step_d"] = (_c_int, [_vp] + [_vp] * 12)
 SIGNATURES["aer_leapfrog_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_vp] * 16)
 SIGNATURES["aer_mass_d"] = (_c_int, [_vp, _vp, _c_int, _c_double, _vp, _vp])
 AER_STEP_SEMI_IMPLICIT, AER_STEP_DIFFUSE, AER_STEP_FILTER = 1, 2, 4
+SIGNATURES["aer_diag_create"] = (_c_int, [_vp, _c_int, ctypes.POINTER(_vp)])
+SIGNATURES["aer_diag_accumulate_d"] = (_c_int, [_vp, _vp, _vp, _vp, _vp])
+SIGNATURES["aer_diag_read"] = (_c_int, [_vp, _ip, _dp])
+SIGNATURES["aer_diag_reset"] = (_c_int, [_vp])
+SIGNATURES["aer_diag_destroy"] = (_c_int, [_vp])
 SIGNATURES["aer_dev_malloc"] = (_c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)])
 SIGNATURES["aer_dev_free"] = (_c_int, [_vp])
 SIGNATURES["aer_memcpy_h2d"] = (_c_int, [_vp, _vp, ctypes.c_size_t])

@@ -6,6 +6,7 @@
 #include "fused_device.h"
 #include "column_device.h"
 #include "step_device.h"
+#include "diag_device.h"
 #include "../../include/aer_sht.h"
 
 #include <algorithm>
@@ -2120,6 +2121,102 @@ int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *m
                        (const double *)h->massg.p, h->nlon, h->nlat, (const double *)h->wfull.p, 9.80665,
                        fix ? 1 : 0, mass_target, h->c00, lnps, mass_out, lnr_out);
     HIP_TRY(hipGetLastError());
+    return AER_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ zonal-mean statistics
+
+struct aer_diag {
+    aer_sht *h = nullptr;
+    int nlev = 0;
+    int nsample = 0;
+    DevBuf sum;
+};
+
+namespace {
+
+long diag_len(const aer_diag *d) { return ((long)aer::DG_NFIELD * d->nlev + 2) * d->h->nlat; }
+
+int diag_zero(aer_diag *d)
+{
+    const long n = diag_len(d);
+    hipLaunchKernelGGL(aer::k_diag_zero, dim3((unsigned)std::min<long>((n + 255) / 256, 1024)), dim3(256), 0,
+                       d->h->stream, (double *)d->sum.p, n);
+    HIP_TRY(hipGetLastError());
+    d->nsample = 0;
+    return AER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_diag_create(aer_sht_t h, int nlev, aer_diag_t *diag)
+{
+    if (!diag) return fail(AER_EINVAL, "aer_diag_create: null output");
+    *diag = nullptr;
+    if (!h || !h->grid_set) return fail(AER_ESTATE, "aer_diag_create: handle without a grid");
+    if (nlev < 1) return fail(AER_EINVAL, "aer_diag_create: nlev < 1");
+    if (h->nlon > 64 * aer::DG_MAXPL) return fail(AER_EINVAL, "aer_diag_create: nlon above 512");
+    aer_diag *d = new aer_diag();
+    d->h = h;
+    d->nlev = nlev;
+    int rc = d->sum.ensure((size_t)diag_len(d) * sizeof(double));
+    if (!rc) rc = diag_zero(d);
+    if (rc) {
+        delete d;
+        return rc;
+    }
+    *diag = d;
+    return AER_OK;
+}
+
+int aer_diag_accumulate_d(aer_diag_t d, const double *u, const double *v, const double *temp, const double *lnps)
+{
+    if (!d) return fail(AER_EINVAL, "aer_diag_accumulate_d: null object");
+    if (!u || !v || !temp || !lnps) return fail(AER_EINVAL, "aer_diag_accumulate_d: null field");
+    aer::DiagArgs A{};
+    A.nlon = d->h->nlon;
+    A.nlat = d->h->nlat;
+    A.nlev = d->nlev;
+    A.u = u;
+    A.v = v;
+    A.temp = temp;
+    A.lnps = lnps;
+    A.sum = (double *)d->sum.p;
+    hipLaunchKernelGGL(aer::k_diag_accumulate, dim3(A.nlat, A.nlev + 1), dim3(64), 0, d->h->stream, A);
+    HIP_TRY(hipGetLastError());
+    d->nsample += 1;
+    return AER_OK;
+}
+
+int aer_diag_read(aer_diag_t d, int *nsample, double *mean)
+{
+    if (!d || !nsample || !mean) return fail(AER_EINVAL, "aer_diag_read: null argument");
+    const long n = diag_len(d);
+    HIP_TRY(hipMemcpyAsync(mean, d->sum.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, d->h->stream));
+    HIP_TRY(hipStreamSynchronize(d->h->stream));
+    *nsample = d->nsample;
+    if (d->nsample > 0) {
+        const double rn = 1.0 / (double)d->nsample;
+        for (long i = 0; i < n; ++i) mean[i] *= rn;
+    }
+    return AER_OK;
+}
+
+int aer_diag_reset(aer_diag_t d)
+{
+    if (!d) return fail(AER_EINVAL, "aer_diag_reset: null object");
+    return diag_zero(d);
+}
+
+int aer_diag_destroy(aer_diag_t d)
+{
+    if (!d) return AER_OK;
+    d->sum.release();
+    delete d;
     return AER_OK;
 }
 

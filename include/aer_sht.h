@@ -339,6 +339,23 @@ int aer_leapfrog_d(aer_semiimp_t si, int flags, double eps_filter, double raw_al
  * NULL). c00 is read back from an analysis of the constant 1 at aer_sht_set_grid. */
 int aer_mass_d(aer_sht_t cfg, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out);
 
+/* ---- zonal-mean statistics: replaces aeros_diag_init / aeros_diag_accumulate /
+ * aeros_diag_finalize's sums (src/aeros_diagnostics.f90:114-276) with FP64 running sums on
+ * the device. One sample (asynchronous on the handle's stream) takes the grid fields
+ * u, v, T [nlev][nlat][nlon] and ln p_s [nlat][nlon] and adds, per (level, latitude), the
+ * instantaneous zonal means [u], [v], [T] and the covariances about them [u'v'], [v'T'],
+ * [(u'^2+v'^2)/2], [T'^2], plus [p_s] (p_s = exp(ln p_s)) and the lowest level's [u].
+ * aer_diag_read synchronizes and returns the time means (sums / samples) as
+ * [7][nlev][nlat] (u, v, T, u'v', v'T', eke, T'^2; level 0 at the top, latitude 0 at the
+ * north) followed by [2][nlat] (p_s, lowest-level u), and the sample count. */
+typedef struct aer_diag *aer_diag_t;
+int aer_diag_create(aer_sht_t cfg, int nlev, aer_diag_t *diag);
+int aer_diag_accumulate_d(aer_diag_t diag, const double *u, const double *v, const double *temp,
+                          const double *lnps_grid);
+int aer_diag_read(aer_diag_t diag, int *nsample, double *mean);
+int aer_diag_reset(aer_diag_t diag);
+int aer_diag_destroy(aer_diag_t diag);
+
 /* ---- device memory for hosts without a HIP binding of their own (Fortran): thin
  * wrappers of hipMalloc / hipFree / synchronous hipMemcpy. No reference counterpart. */
 int aer_dev_malloc(size_t bytes, void **ptr);

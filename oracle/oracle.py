@@ -430,3 +430,37 @@ def mass_fix(o, lnps, target):
     lnr = np.log(target / mass_of(o, lnps))
     lnps[0] += lnr * c00(o)
     return lnps, lnr
+
+
+def zonal_sample(u, v, temp, lnps_g):
+    """One sample of aeros_diag_accumulate (src/aeros_diagnostics.f90:153-233) in plain loops
+    over longitude, FP64: per (level, latitude) the zonal means [u], [v], [T] and the
+    covariances about the instantaneous means [u'v'], [v'T'], [(u'^2+v'^2)/2], [T'^2];
+    per latitude [p_s] (p_s = exp(ln p_s)) and the lowest level's [u]. Arrays are
+    (nlev, nlat, nlon) and (nlat, nlon); returns a dict of (nlev, nlat) / (nlat,) arrays."""
+    nlev, nlat, nlon = u.shape
+    rn = 1.0 / nlon
+    out = {k: np.zeros((nlev, nlat)) for k in ("u", "v", "temp", "uv", "vt", "eke", "tvar")}
+    for k in range(nlev):
+        for j in range(nlat):
+            um = vm = tm = 0.0
+            for i in range(nlon):
+                um += u[k, j, i]
+                vm += v[k, j, i]
+                tm += temp[k, j, i]
+            um *= rn
+            vm *= rn
+            tm *= rn
+            uvm = vtm = ekem = tvm = 0.0
+            for i in range(nlon):
+                up, vp, tp = u[k, j, i] - um, v[k, j, i] - vm, temp[k, j, i] - tm
+                uvm += up * vp
+                vtm += vp * tp
+                ekem += 0.5 * (up * up + vp * vp)
+                tvm += tp * tp
+            out["u"][k, j], out["v"][k, j], out["temp"][k, j] = um, vm, tm
+            out["uv"][k, j], out["vt"][k, j] = uvm * rn, vtm * rn
+            out["eke"][k, j], out["tvar"][k, j] = ekem * rn, tvm * rn
+    out["ps"] = np.array([sum(np.exp(lnps_g[j, i]) for i in range(nlon)) * rn for j in range(nlat)])
+    out["u_sfc"] = np.array([sum(u[nlev - 1, j, i] for i in range(nlon)) * rn for j in range(nlat)])
+    return out

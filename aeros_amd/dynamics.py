@@ -21,6 +21,7 @@ from .spectral import AerosSht
 from .step import SemiImplicit, mass
 
 HS_KEYS = ("vor", "div", "temp", "lnps")
+CKPT_VERSION = 1
 
 
 class Stepper:
@@ -43,6 +44,7 @@ class Stepper:
         self.phis = None
         self.nstep = 0
         self.mass_target = None
+        self.lnr_cum = None       # running sum of the mass fixer's ln r (device scalar)
         self.now = self.old = None
 
     def set_state(self, vor, div, temp, lnps):
@@ -72,7 +74,65 @@ class Stepper:
                                eps_filter=eps, raw_alpha=self.raw_alpha)
         self.old, self.now = now, new
         if self.mass_fixer:
-            mass(s, self.now["lnps"], fix=True, target=self.mass_target)
+            _, lnr = mass(s, self.now["lnps"], fix=True, target=self.mass_target)
+            self.lnr_cum = lnr if self.lnr_cum is None else self.lnr_cum + lnr
         self.nstep += 1
         self.last_grid = g
         return self.now
+
+    # ------------------------------------------------------------ checkpoint / restart
+    def checkpoint(self):
+        """The integrator state a resumed run needs to continue bit-identically, as in
+        aeros_timestep_write_restart (src/dynamics/aeros_timestep.f90:1303-1337) for the dry
+        core: both leapfrog levels X^(n-1) and X^n, nstep, the mass-fixer target and its
+        accumulated ln r. Device copies; grid fields are recomputed every step and are not
+        part of it."""
+        if self.now is None:
+            raise RuntimeError("no state to checkpoint")
+        return {"version": CKPT_VERSION, "nstep": self.nstep,
+                "old": None if self.old is None else {k: v.clone() for k, v in self.old.items()},
+                "now": {k: v.clone() for k, v in self.now.items()},
+                "mass_target": self.mass_target,
+                "lnr_cum": None if self.lnr_cum is None else self.lnr_cum.clone(),
+                "shape": (self.nlev, self.sht.nlm, self.sht.nlat, self.sht.nlon)}
+
+    def restore(self, ck):
+        """Resume from checkpoint(); the shapes must match this stepper's."""
+        if ck.get("version") != CKPT_VERSION:
+            raise ValueError("checkpoint version mismatch")
+        if tuple(ck["shape"]) != (self.nlev, self.sht.nlm, self.sht.nlat, self.sht.nlon):
+            raise ValueError(f"checkpoint of shape {tuple(ck['shape'])} does not fit this stepper")
+        self.now = {k: v.clone() for k, v in ck["now"].items()}
+        self.old = None if ck["old"] is None else {k: v.clone() for k, v in ck["old"].items()}
+        self.nstep = int(ck["nstep"])
+        self.mass_target = ck["mass_target"]
+        self.lnr_cum = None if ck["lnr_cum"] is None else ck["lnr_cum"].clone()
+
+
+def save_checkpoint(ck, path):
+    """A checkpoint to a .npz file: spectral levels as complex128 arrays, scalars and the
+    shape (nlev, nlm, nlat, nlon) as metadata. Loaded without pickle."""
+    import numpy as np
+    arrs = {"version": np.array(ck["version"]), "nstep": np.array(ck["nstep"]),
+            "shape": np.array(ck["shape"], dtype=np.int64),
+            "mass_target": np.array(np.nan if ck["mass_target"] is None else ck["mass_target"]),
+            "lnr_cum": np.array(np.nan if ck["lnr_cum"] is None else float(ck["lnr_cum"]))}
+    for lev in ("old", "now"):
+        if ck[lev] is not None:
+            for k, v in ck[lev].items():
+                arrs[f"{lev}_{k}"] = v.cpu().numpy()
+    np.savez(path, **arrs)
+
+
+def load_checkpoint(path, device="cuda"):
+    import numpy as np
+    z = np.load(path, allow_pickle=False)
+    levs = {}
+    for lev in ("old", "now"):
+        keys = [k for k in HS_KEYS if f"{lev}_{k}" in z.files]
+        levs[lev] = {k: torch.from_numpy(z[f"{lev}_{k}"]).to(device) for k in keys} if keys else None
+    mt, lc = float(z["mass_target"]), float(z["lnr_cum"])
+    return {"version": int(z["version"]), "nstep": int(z["nstep"]), "old": levs["old"], "now": levs["now"],
+            "mass_target": None if np.isnan(mt) else mt,
+            "lnr_cum": None if np.isnan(lc) else torch.tensor(lc, dtype=torch.float64, device=device),
+            "shape": tuple(int(x) for x in z["shape"])}

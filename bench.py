@@ -349,6 +349,113 @@ def ensemble_config2(torch, precision, steps=50, warmup=5, path="auto"):
             "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
+def stress_T106(torch, members=32, nlev=30, steps=5, warmup=1):
+    """The stress configuration's transform work: the S_ref set (11L+4 transforms) of 32
+    T106L30 members (Gaussian 160 x 320) in one compiled plan, split-FP32 MFMA Legendre, and
+    the same plan in FP64. Reports sets/s, scalar-equivalent transforms/s, the HBM rate of
+    the algorithmic traffic, device memory in use, and the per-field error of the split
+    form's outputs against FP64 for the whole ensemble."""
+    from aeros_amd import _lib as L
+    from aeros_amd.seeding import red_spectra, red_spectrum
+    from aeros_amd.spectral import AerosSht
+    T = 106
+    free0, total = torch.cuda.mem_get_info()
+    res = {"workload": f"{members} x T{T}L{nlev} (Gaussian 160 x 320), S_ref set each, one compiled plan",
+           "members": members}
+    outs = {}
+    a = 1.0 / 6.371e6
+    P = lambda t: t.data_ptr()
+    J = lambda kind, n, ins, outs_, md=(0, 0, 0), sc=(0.0, 0.0): L.AerJob(
+        kind, n, *(list(map(P, ins)) + [0] * (3 - len(ins))), *(list(map(P, outs_)) + [0] * (2 - len(outs_))),
+        0, 0, *md, *sc, 0)
+    s = AerosSht(T)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    ins, nbytes = [], 0
+    for mbr in range(members):
+        vor = dev(red_spectra(T, nlev, 4, member=mbr, field0=0, zero_l0=True) * 1e-5)
+        div = dev(red_spectra(T, nlev, 4, member=mbr, field0=100, zero_l0=True) * 1e-6)
+        tem = dev(red_spectra(T, nlev, 4, member=mbr, field0=200))
+        lnp = dev(red_spectrum(T, 4000 + 300 + 100000 * mbr) * 1e-2)
+        gen = torch.Generator(device="cuda").manual_seed(mbr)
+        ae, an, ek, dt = (torch.randn((nlev, s.nlat, s.nlon), dtype=torch.float64, device="cuda", generator=gen)
+                          for _ in range(4))
+        dl = torch.randn((1, s.nlat, s.nlon), dtype=torch.float64, device="cuda", generator=gen)
+        ins.append((vor, div, tem, lnp, ae, an, ek, dt, dl))
+        nbytes += sref_bytes(s.nlon, s.nlat, s.nlm, nlev)
+    for prec in ("fp64", "split"):
+        s.set_precision(prec)
+        G = lambda n=nlev: torch.empty((n, s.nlat, s.nlon), dtype=torch.float64, device="cuda")
+        S = lambda n=nlev: torch.empty((n, s.nlm), dtype=torch.complex128, device="cuda")
+        cfgs, jobs, o_all = [], [], []
+        for (vor, div, tem, lnp, ae, an, ek, dt, dl) in ins:
+            o = {"u": G(), "v": G(), "vor_g": G(), "div_g": G(), "t_g": G(), "dtdx": G(), "dtdy": G(),
+                 "lnps_g": G(1), "dlnpsdx": G(1), "dlnpsdy": G(1), "tnd_vor": S(), "tnd_div": S(), "tnd_temp": S(),
+                 "tnd_lnps": S(1)}
+            mj = [J(L.AER_JOB_SYNTH_SCALAR, 1, [lnp], [o["lnps_g"]]),
+                  J(L.AER_JOB_SYNTH_GRAD, 1, [lnp], [o["dlnpsdy"], o["dlnpsdx"]], sc=(-a, a)),
+                  J(L.AER_JOB_SYNTH_VECTOR, nlev, [div, vor], [o["v"], o["u"]],
+                    (L.AER_MODE_S_FROM_DIV, L.AER_MODE_T_FROM_VOR, 0), (-1.0, 1.0)),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [vor], [o["vor_g"]]),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [div], [o["div_g"]]),
+                  J(L.AER_JOB_SYNTH_SCALAR, nlev, [tem], [o["t_g"]]),
+                  J(L.AER_JOB_SYNTH_GRAD, nlev, [tem], [o["dtdy"], o["dtdx"]], sc=(-a, a)),
+                  J(L.AER_JOB_ANAL_VECTOR, nlev, [an, ae, ek], [o["tnd_div"], o["tnd_vor"]],
+                    (L.AER_MODE_DIV_FROM_S, L.AER_MODE_VOR_FROM_T, L.AER_MODE_NEG_LAP), (-1.0, 1.0)),
+                  J(L.AER_JOB_ANAL_SCALAR, nlev, [dt], [o["tnd_temp"]]),
+                  J(L.AER_JOB_ANAL_SCALAR, 1, [dl], [o["tnd_lnps"]])]
+            cfgs += [s._h] * len(mj)
+            jobs += mj
+            o_all.append(o)
+        n = len(jobs)
+        plan = ctypes.c_void_p()
+        L.check(L.lib().aer_plan_create(n, (ctypes.c_void_p * n)(*cfgs), (L.AerJob * n)(*jobs), ctypes.byref(plan)))
+        st = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        run = lambda: L.check(L.lib().aer_plan_run(plan, st()))
+        ms = time_steps(torch, run, steps, warmup)
+        free1, _ = torch.cuda.mem_get_info()
+        L.lib().aer_plan_destroy(plan)
+        ntr = members * (11 * nlev + 4)
+        res[prec] = {"ms_per_ensemble_step": round(ms, 3), "sets_per_s": round(members * 1000.0 / ms, 1),
+                     "scalar_eq_transforms_per_s": round(ntr * 1000.0 / ms, 0),
+                     "hbm_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "device_memory_in_use_GB": round((total - free1) / 1e9, 2)}
+        outs[prec] = o_all
+    errs = {}
+    for k in outs["fp64"][0]:
+        errs[k] = max(rel_err_any(a_[k], b_[k]) for a_, b_ in zip(outs["split"], outs["fp64"]))
+    res["split_rel_err_vs_fp64_per_field"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+    res["split_max_rel_err"] = max(errs.values())
+    res["alg_bytes_per_step"] = int(nbytes)
+    s.set_precision("fp64")
+    return res
+
+
+def rel_err_any(a, ref):
+    """max|a - ref| / max|ref| over a whole tensor (the reference's measure)."""
+    a, ref = a.cpu().numpy(), ref.cpu().numpy()
+    return float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def held_suarez_climate(torch, trunc=31, days=1200.0):
+    """The Held-Suarez protocol of drivers/held_suarez.f90 at T31L20: 1200 days, the last 1000
+    averaged, in FP64, with FP32 transforms, and an FP64 twin whose seed differs by 1e-12
+    relative (its difference from the FP64 run is the sampling noise of the window)."""
+    from aeros_amd.held_suarez import compare, run
+    out = {}
+    res = {}
+    for name, prec, amp in (("fp64", "fp64", 1.0), ("fp32", "fp32", 1.0), ("fp64_twin", "fp64", 1.0 + 1e-12)):
+        sm, d, info = run(trunc=trunc, nlev=20, dt=1800.0, days=days, avg_start=200.0, precision=prec, perturb=amp)
+        res[name] = d
+        out[name] = {"summary": {k: round(v, 3) for k, v in sm.items()}, "wall_s": info["wall_s"],
+                     "ms_per_step": info["ms_per_step"]}
+    out["fp32_vs_fp64"] = {k: round(v, 4) for k, v in compare(res["fp32"], res["fp64"]).items()}
+    out["fp64_twin_vs_fp64"] = {k: round(v, 4) for k, v in compare(res["fp64_twin"], res["fp64"]).items()}
+    out["published_T31L20"] = {"u_max": 35.3, "u_max_lat": -42.7, "vt_max": 18.7, "uv_max": 74.4, "eke_max": 268.0,
+                               "t_sfc_equator": 306.8, "t_sfc_pole": 263.0,
+                               "source": "reference docs/m1_results.md, T31L20 column (SH jet)"}
+    return out
+
+
 def dynamics_step(torch, precision, steps=20, warmup=3, path="staged"):
     """Whole Held-Suarez dynamics steps at T42L20 (transform halves, column terms +
     forcing, semi-implicit leapfrog with diffusion and filter, mass fixer), eager
@@ -399,8 +506,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hs", action="store_true", help="skip the 1200-day Held-Suarez climate runs (~25 s)")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 32 x T106L30 stress plan")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--only", choices=["set", "ensemble", "step"], default=None,
+    ap.add_argument("--only", choices=["set", "ensemble", "step", "stress", "hs"], default=None,
                     help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms), only the "
                          "ensemble, or only whole Held-Suarez dynamics steps")
     ap.add_argument("--path", choices=["auto", "tiled", "staged"], default="auto",
@@ -429,6 +538,12 @@ def main():
         print(json.dumps({"held_suarez_step_ms": dynamics_step(torch, args.precision, args.steps, args.warmup,
                                                                args.path),
                           "precision": args.precision, "kernel_path": args.path}))
+        return
+    if args.only == "stress":
+        print(json.dumps({"stress_T106L30x32": stress_T106(torch, steps=args.steps, warmup=args.warmup)}))
+        return
+    if args.only == "hs":
+        print(json.dumps({"held_suarez_T31L20_1200d": held_suarez_climate(torch)}))
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup,
@@ -573,6 +688,10 @@ def main():
                 "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
         "ms_per_step_fp64": dynamics_step(torch, "fp64", path=args.path),
         "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path)}
+    if not args.no_hs:
+        line["held_suarez_T31L20_1200d"] = held_suarez_climate(torch)
+    if not args.no_stress:
+        line["stress_T106L30x32"] = stress_T106(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

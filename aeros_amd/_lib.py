@@ -100,6 +100,8 @@ SIGNATURES["aer_synth_grad_adj_d"] = (_c_int, [_vp, _c_int, _vp, _vp, _c_long, _
 SIGNATURES["aer_vgrid_create"] = (_c_int, [_c_int, _dp, _dp, ctypes.POINTER(_vp)])
 SIGNATURES["aer_vgrid_destroy"] = (_c_int, [_vp])
 SIGNATURES["aer_column_terms_d"] = (_c_int, [_vp, _vp, _c_int] + [_vp] * 16)
+SIGNATURES["aer_column_terms_dteq_d"] = (_c_int, [_vp, _vp] + [_vp] * 17)
+SIGNATURES["aer_column_terms_adj_d"] = (_c_int, [_vp, _vp, _c_int] + [_vp] * 28)
 SIGNATURES["aer_semiimp_create"] = (_c_int, [_vp, _vp, _c_double, _c_double, _c_double, _c_double, ctypes.POINTER(_vp)])
 SIGNATURES["aer_semiimp_set_step"] = (_c_int, [_vp, _c_double])
 SIGNATURES["aer_semiimp_set_diffusion"] = (_c_int, [_vp, _c_double, _c_double, _c_int])
@@ -109,6 +111,7 @@ SIGNATURES["aer_semiimp_linear_d"] = (_c_int, [_vp] + [_vp] * 6)
 SIGNATURES["aer_semiimp_step_d"] = (_c_int, [_vp] + [_vp] * 12)
 SIGNATURES["aer_leapfrog_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_vp] * 16)
 SIGNATURES["aer_mass_d"] = (_c_int, [_vp, _vp, _c_int, _c_double, _vp, _vp])
+SIGNATURES["aer_leapfrog_adj_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_vp] * 20)
 AER_STEP_SEMI_IMPLICIT, AER_STEP_DIFFUSE, AER_STEP_FILTER = 1, 2, 4
 SIGNATURES["aer_diag_create"] = (_c_int, [_vp, _c_int, ctypes.POINTER(_vp)])
 SIGNATURES["aer_diag_accumulate_d"] = (_c_int, [_vp, _vp, _vp, _vp, _vp])

@@ -65,7 +65,7 @@ class VGrid:
             self._h = None
 
 
-def column_terms(sht, vg, grid, phis=None, held_suarez=False, out=None):
+def column_terms(sht, vg, grid, phis=None, held_suarez=False, out=None, dteq=None):
     """Grid right-hand sides of every column from the dict that
     AerosSht.tendency_grid_transforms returns. Returns ae, an, ek, dtdt (3-D) and
     dlnpsdt (2-D) as a dict; inputs are left unchanged."""
@@ -84,10 +84,36 @@ def column_terms(sht, vg, grid, phis=None, held_suarez=False, out=None):
         out = {k: torch.empty_like(u) for k in ("ae", "an", "ek", "dtdt")}
         out["dlnpsdt"] = torch.empty_like(grid["lnps_g"])
     sht._bind_stream()
-    check(lib().aer_column_terms_d(
-        sht._h, vg._h, 1 if held_suarez else 0,
-        *[grid[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
-                                       "lnps_g", "dlnpsdx", "dlnpsdy")],
-        phis.data_ptr() if phis is not None else None,
-        *[out[k].data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")]))
+    ins = [grid[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
+                                         "lnps_g", "dlnpsdx", "dlnpsdy")]
+    outs = [out[k].data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")]
+    ph = phis.data_ptr() if phis is not None else None
+    if dteq is not None:
+        # Held-Suarez with T_eq = formula + dteq ([nlev, nlat]; the adjoint's control variable)
+        if not held_suarez:
+            raise ValueError("dteq needs the Held-Suarez forcing")
+        if tuple(dteq.shape) != (nlev, sht.nlat) or not (dteq.is_cuda and dteq.dtype == torch.float64):
+            raise ValueError("dteq: expected a float64 CUDA tensor of shape (nlev, nlat)")
+        check(lib().aer_column_terms_dteq_d(sht._h, vg._h, dteq.contiguous().data_ptr(), *ins, ph, *outs))
+    else:
+        check(lib().aer_column_terms_d(sht._h, vg._h, 1 if held_suarez else 0, *ins, ph, *outs))
     return out
+
+
+def column_terms_adjoint(sht, vg, grid, cbar, phis=None, held_suarez=False, dteq=None, want_teq=False):
+    """Adjoint of column_terms at the inputs `grid`: from the adjoints of ae, an, ek, dtdt,
+    dlnpsdt (dict cbar) to those of the ten grid inputs, and, with want_teq, the sensitivity
+    to dteq per (level, latitude) (summed over longitudes)."""
+    import torch
+    keys = ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy", "lnps_g", "dlnpsdx", "dlnpsdy")
+    g = {k: torch.empty_like(grid[k]) for k in keys}
+    gteq = torch.empty_like(grid["u"]) if want_teq else None
+    sht._bind_stream()
+    check(lib().aer_column_terms_adj_d(
+        sht._h, vg._h, 1 if held_suarez else 0, dteq.contiguous().data_ptr() if dteq is not None else None,
+        *[grid[k].data_ptr() for k in keys], phis.data_ptr() if phis is not None else None,
+        *[cbar[k].contiguous().data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")],
+        *[g[k].data_ptr() for k in keys], gteq.data_ptr() if gteq is not None else None))
+    if want_teq:
+        return g, gteq.sum(dim=2)
+    return g

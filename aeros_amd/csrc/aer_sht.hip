@@ -5,6 +5,7 @@
 #include "sht_device.h"
 #include "fused_device.h"
 #include "column_device.h"
+#include "column_adj.h"
 #include "step_device.h"
 #include "diag_device.h"
 #include "../../include/aer_sht.h"
@@ -1762,19 +1763,20 @@ int aer_vgrid_destroy(aer_vgrid_t vg)
     return AER_OK;
 }
 
-int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
-                       const double *u, const double *v, const double *vor_g, const double *div_g,
-                       const double *t_g, const double *dtdx, const double *dtdy,
-                       const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
-                       const double *phis,
-                       double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
+}  // extern "C"
+
+namespace {
+
+int column_args(aer_sht *h, aer_vgrid *vg, int held_suarez, const double *u, const double *v, const double *vor_g,
+                const double *div_g, const double *t_g, const double *dtdx, const double *dtdy, const double *lnps_g,
+                const double *dlnpsdx, const double *dlnpsdy, const double *phis, const double *dteq,
+                aer::ColumnArgs &C, const char *who)
 {
-    if (!h || !vg) return fail(AER_EINVAL, "aer_column_terms_d: null handle");
-    if (!h->grid_set) return fail(AER_ESTATE, "aer_column_terms_d: handle without a grid");
-    if (!u || !v || !vor_g || !div_g || !t_g || !dtdx || !dtdy || !lnps_g || !dlnpsdx || !dlnpsdy ||
-        !ae || !an || !ek || !dtdt || !dlnpsdt)
-        return fail(AER_EINVAL, "aer_column_terms_d: missing array");
-    aer::ColumnArgs C{};
+    if (!h || !vg) return fail(AER_EINVAL, std::string(who) + ": null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, std::string(who) + ": handle without a grid");
+    if (!u || !v || !vor_g || !div_g || !t_g || !dtdx || !dtdy || !lnps_g || !dlnpsdx || !dlnpsdy)
+        return fail(AER_EINVAL, std::string(who) + ": missing array");
+    C = aer::ColumnArgs{};
     C.nlon = h->nlon;
     C.ncol = h->nlon * h->nlat;
     C.nlev = vg->nlev;
@@ -1785,10 +1787,79 @@ int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
     C.sinlat = (const double *)h->sinlat.p;
     C.u = u; C.v = v; C.vor = vor_g; C.div = div_g; C.t = t_g; C.dtdx = dtdx; C.dtdy = dtdy;
     C.lnps = lnps_g; C.dlnpsdx = dlnpsdx; C.dlnpsdy = dlnpsdy; C.phis = phis;
+    C.dteq = dteq;
+    return AER_OK;
+}
+
+int column_launch(aer_sht *h, aer::ColumnArgs &C, double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt,
+                  const char *who)
+{
+    if (!ae || !an || !ek || !dtdt || !dlnpsdt) return fail(AER_EINVAL, std::string(who) + ": missing array");
     C.ae = ae; C.an = an; C.ek = ek; C.dtdt = dtdt; C.dlnpsdt = dlnpsdt;
     const int nb = (C.ncol + AER_COL_CB - 1) / AER_COL_CB;
     const size_t lds = (size_t)(2 * (C.nlev + 1) + (7 * C.nlev + 1) * AER_COL_CB) * sizeof(double);
     hipLaunchKernelGGL(aer::k_column_terms, dim3(nb), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
+    HIP_TRY(hipGetLastError());
+    return AER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
+                       const double *u, const double *v, const double *vor_g, const double *div_g,
+                       const double *t_g, const double *dtdx, const double *dtdy,
+                       const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                       const double *phis,
+                       double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
+{
+    aer::ColumnArgs C;
+    int rc = column_args(h, vg, held_suarez, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis,
+                         nullptr, C, "aer_column_terms_d");
+    if (rc) return rc;
+    return column_launch(h, C, ae, an, ek, dtdt, dlnpsdt, "aer_column_terms_d");
+}
+
+int aer_column_terms_dteq_d(aer_sht_t h, aer_vgrid_t vg, const double *dteq,
+                            const double *u, const double *v, const double *vor_g, const double *div_g,
+                            const double *t_g, const double *dtdx, const double *dtdy,
+                            const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                            const double *phis,
+                            double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
+{
+    aer::ColumnArgs C;
+    int rc = column_args(h, vg, 1, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis, dteq, C,
+                         "aer_column_terms_dteq_d");
+    if (rc) return rc;
+    return column_launch(h, C, ae, an, ek, dtdt, dlnpsdt, "aer_column_terms_dteq_d");
+}
+
+int aer_column_terms_adj_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez, const double *dteq,
+                           const double *u, const double *v, const double *vor_g, const double *div_g,
+                           const double *t_g, const double *dtdx, const double *dtdy,
+                           const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                           const double *phis,
+                           const double *b_ae, const double *b_an, const double *b_ek, const double *b_dtdt,
+                           const double *b_dlnpsdt,
+                           double *g_u, double *g_v, double *g_vor, double *g_div, double *g_t,
+                           double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx, double *g_dlnpsdy,
+                           double *g_teq)
+{
+    aer::ColumnAdjArgs X{};
+    int rc = column_args(h, vg, held_suarez, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis,
+                         dteq, X.f, "aer_column_terms_adj_d");
+    if (rc) return rc;
+    if (vg->nlev > AER_ADJ_MAXLEV) return fail(AER_EINVAL, "aer_column_terms_adj_d: nlev above 64");
+    if (!b_ae || !b_an || !b_ek || !b_dtdt || !b_dlnpsdt || !g_u || !g_v || !g_vor || !g_div || !g_t || !g_dtdx ||
+        !g_dtdy || !g_lnps || !g_dlnpsdx || !g_dlnpsdy)
+        return fail(AER_EINVAL, "aer_column_terms_adj_d: missing array");
+    X.dteq = dteq;
+    X.bae = b_ae; X.ban = b_an; X.bek = b_ek; X.bdtdt = b_dtdt; X.bdlnpsdt = b_dlnpsdt;
+    X.gu = g_u; X.gv = g_v; X.gvor = g_vor; X.gdiv = g_div; X.gt = g_t; X.gdtdx = g_dtdx; X.gdtdy = g_dtdy;
+    X.glnps = g_lnps; X.gdlnpsdx = g_dlnpsdx; X.gdlnpsdy = g_dlnpsdy; X.gteq = g_teq;
+    const int nb = (X.f.ncol + 63) / 64;
+    hipLaunchKernelGGL(aer::k_column_terms_adj, dim3(nb), dim3(64), 0, h->stream, X);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
@@ -2101,6 +2172,58 @@ int aer_leapfrog_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alp
     return launch_step(s, f, eps_filter, raw_alpha, old_vor, old_div, old_temp, old_lnps,
                        now_vor, now_div, now_temp, now_lnps, tnd_vor, tnd_div, tnd_temp, tnd_lnps,
                        new_vor, new_div, new_temp, new_lnps);
+}
+
+int aer_leapfrog_adj_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alpha,
+                       const double *bn_vor, const double *bn_div, const double *bn_temp, const double *bn_lnps,
+                       const double *bc_vor, const double *bc_div, const double *bc_temp, const double *bc_lnps,
+                       double *go_vor, double *go_div, double *go_temp, double *go_lnps,
+                       double *gc_vor, double *gc_div, double *gc_temp, double *gc_lnps,
+                       double *gt_vor, double *gt_div, double *gt_temp, double *gt_lnps)
+{
+    if (!s) return fail(AER_EINVAL, "aer_leapfrog_adj_d: null object");
+    if (flags & ~(AER_STEP_SEMI_IMPLICIT | AER_STEP_DIFFUSE | AER_STEP_FILTER))
+        return fail(AER_EINVAL, "aer_leapfrog_adj_d: unknown flag");
+    if ((flags & AER_STEP_DIFFUSE) && !s->diff)
+        return fail(AER_ESTATE, "aer_leapfrog_adj_d: diffusion requested before aer_semiimp_set_diffusion");
+    const void *ptrs[] = {bn_vor, bn_div, bn_temp, bn_lnps, bc_vor, bc_div, bc_temp, bc_lnps, go_vor, go_div, go_temp,
+                          go_lnps, gc_vor, gc_div, gc_temp, gc_lnps, gt_vor, gt_div, gt_temp, gt_lnps};
+    for (const void *p : ptrs)
+        if (!p) return fail(AER_EINVAL, "aer_leapfrog_adj_d: missing array");
+    aer::StepAdjArgs A{};
+    A.nlev = s->nlev;
+    A.lmax = s->lmax;
+    A.nlm = s->h->nlm;
+    A.flags = 0;
+    if (flags & AER_STEP_SEMI_IMPLICIT) A.flags |= aer::ST_SEMI;
+    if (flags & AER_STEP_DIFFUSE) A.flags |= aer::ST_DIFF;
+    if ((flags & AER_STEP_FILTER) && eps_filter > 0.0) A.flags |= aer::ST_RAW;
+    A.h = s->dt;
+    A.a1 = s->alpha;
+    A.astep = s->alpha * s->dt;
+    A.hh_old = s->alpha * (1.0 - s->alpha) * s->dt * s->dt;
+    A.half_nu = 0.5 * eps_filter;
+    A.raw_a = raw_alpha;
+    A.raw_b = 1.0 - raw_alpha;
+    A.G = (const double *)s->dG.p;
+    A.tau = (const double *)s->dtau.p;
+    A.W = (const double *)s->dW.p;
+    A.nu = (const double *)s->dnu.p;
+    A.pcoef = (const double *)s->dpcoef.p;
+    A.Minv = (const double *)s->dMinv.p;
+    A.fvt = (const double *)s->dfvt.p;
+    A.fdiv = (const double *)s->dfdiv.p;
+    A.task_l = (const int *)s->dtl.p;
+    A.task_c0 = (const int *)s->dtc.p;
+    A.bnvor = bn_vor; A.bndiv = bn_div; A.bntemp = bn_temp; A.bnlnps = bn_lnps;
+    A.bcvor = bc_vor; A.bcdiv = bc_div; A.bctemp = bc_temp; A.bclnps = bc_lnps;
+    A.govor = go_vor; A.godiv = go_div; A.gotemp = go_temp; A.golnps = go_lnps;
+    A.gcvor = gc_vor; A.gcdiv = gc_div; A.gctemp = gc_temp; A.gclnps = gc_lnps;
+    A.gtvor = gt_vor; A.gtdiv = gt_div; A.gttemp = gt_temp; A.gtlnps = gt_lnps;
+    const size_t lds = (size_t)(5 * s->nlev + 3) * AER_SI_CB * sizeof(double);
+    hipLaunchKernelGGL(aer::k_step_adj, dim3(s->nblk), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
+    HIP_TRY(hipGetLastError());
+    return AER_OK;
 }
 
 int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out)

@@ -38,6 +38,7 @@ struct ColumnArgs {
     const double *lnps, *dlnpsdx, *dlnpsdy, *phis;       // [ncol]; phis may be null
     double *ae, *an, *ek, *dtdt;    // [nlev][ncol]
     double *dlnpsdt;                // [ncol]
+    const double *dteq;             // optional additive Held-Suarez T_eq [nlev][nlat], or null
 };
 
 // Per-level column geometry: dp, alpha, dlnp (with the zero-top convention of
@@ -180,6 +181,7 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C)
             const double pfac = pf / kHsP0;
             double teq = (kHsTeq0 - kHsDTy * s2 - kHsDThz * log(pfac) * c2) * pow(pfac, kKappa);
             teq = teq > kHsTmin ? teq : kHsTmin;
+            if (C.dteq) teq += C.dteq[(long)k * (C.ncol / C.nlon) + cc / C.nlon];
             double hf = (sig - kHsSigB) / (1.0 - kHsSigB);
             hf = hf > 0.0 ? hf : 0.0;
             const double kt = kHsKa + (kHsKs - kHsKa) * hf * c4;

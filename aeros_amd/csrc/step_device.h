@@ -267,3 +267,213 @@ __global__ __launch_bounds__(256) void k_mass(const double *lnps_g, int nlon, in
 }
 
 }  // namespace aer
+
+namespace aer {
+
+// Adjoint of k_step (the transpose of the linear map (X^(n-1), X^n, R) -> (X^(n+1),
+// filtered X^n) that one leapfrog step applies at fixed operators). No reference
+// counterpart. Same block shape as k_step: one degree l, 16 real columns, 16 level
+// lanes; every stage is a transposed small matrix-vector product over LDS columns.
+struct StepAdjArgs {
+    int nlev, lmax, nlm, flags;
+    double h, a1, astep, hh_old, half_nu, raw_a, raw_b;
+    const double *G, *tau, *W, *nu, *pcoef, *Minv, *fvt, *fdiv;
+    const int *task_l, *task_c0;
+    const double *bnvor, *bndiv, *bntemp, *bnlnps;   // adjoint of X^(n+1)
+    const double *bcvor, *bcdiv, *bctemp, *bclnps;   // adjoint of the filtered X^n
+    double *govor, *godiv, *gotemp, *golnps;         // adjoint of X^(n-1)
+    double *gcvor, *gcdiv, *gctemp, *gclnps;         // adjoint of X^n (the input, before the filter)
+    double *gtvor, *gtdiv, *gttemp, *gtlnps;         // adjoint of the tendency R(X^n)
+};
+
+__global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S)
+{
+    extern __shared__ double lds[];
+    const int n = S.nlev;
+    const int c = threadIdx.x % AER_SI_CB, kl = threadIdx.x / AER_SI_CB;
+    const int l = S.task_l[blockIdx.x], cg = S.task_c0[blockIdx.x] + c;
+    const bool live = cg < 2 * (l + 1);
+    const int m = live ? cg / 2 : 0, part = cg & 1;
+    const int lm = m * (2 * S.lmax + 3 - m) / 2 + (l - m);
+    const long q = 2L * lm + part;
+    const long ls = 2L * S.nlm;
+    const double lam = (double)(l * (l + 1)) / (6.371e6 * 6.371e6);
+    constexpr int NR = AER_SI_MAXLEV / AER_SI_KL;
+    const int NC = AER_SI_CB;
+    double *TRAW = lds, *DNRT = TRAW + n * NC, *RHSB = DNRT + n * NC, *LDB = RHSB + n * NC, *LTB = LDB + n * NC;
+    double *PC = LTB + n * NC;   // per column: praw, ps_b, lp_b
+    const bool semi = S.flags & ST_SEMI, diff = S.flags & ST_DIFF, raw = S.flags & ST_RAW;
+    // per own level: adjoint pieces kept in registers
+    double od[NR], ot[NR], nd[NR], nt[NR], rd[NR], rt[NR], draw[NR];
+    // 0. filter and diffusion transposed, vorticity finished
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int k = kl + r * AER_SI_KL;
+        od[r] = ot[r] = nd[r] = nt[r] = rd[r] = rt[r] = draw[r] = 0.0;
+        if (k >= n) break;
+        const long x = k * ls + q;
+        double bnd = live ? S.bndiv[x] : 0.0, bnt = live ? S.bntemp[x] : 0.0, bnv = live ? S.bnvor[x] : 0.0;
+        double bcd = live ? S.bcdiv[x] : 0.0, bct = live ? S.bctemp[x] : 0.0, bcv = live ? S.bcvor[x] : 0.0;
+        double bov = 0.0;
+        if (raw) {
+            double d = S.raw_a * bcd - S.raw_b * bnd;
+            bcd -= 2.0 * S.half_nu * d;
+            bnd += S.half_nu * d;
+            od[r] += S.half_nu * d;
+            d = S.raw_a * bct - S.raw_b * bnt;
+            bct -= 2.0 * S.half_nu * d;
+            bnt += S.half_nu * d;
+            ot[r] += S.half_nu * d;
+            d = S.raw_a * bcv - S.raw_b * bnv;
+            bcv -= 2.0 * S.half_nu * d;
+            bnv += S.half_nu * d;
+            bov += S.half_nu * d;
+        }
+        if (diff) {
+            const double fv = S.fvt[k * (S.lmax + 1) + l], fd = S.fdiv[k * (S.lmax + 1) + l];
+            bnv *= fv;
+            bnd *= fd;
+            bnt *= fv;
+        }
+        nd[r] = bcd;
+        nt[r] = bct;
+        draw[r] = bnd;
+        TRAW[k * NC + c] = bnt;
+        if (live) {
+            S.govor[x] = bov + bnv;
+            S.gtvor[x] = S.h * bnv;
+            S.gcvor[x] = bcv;
+        }
+    }
+    double bp_o = 0.0, bp_c = 0.0;
+    if (kl == 0) {
+        double bnp = live ? S.bnlnps[q] : 0.0, bcp = live ? S.bclnps[q] : 0.0;
+        if (raw) {
+            const double d = S.raw_a * bcp - S.raw_b * bnp;
+            bcp -= 2.0 * S.half_nu * d;
+            bnp += S.half_nu * d;
+            bp_o += S.half_nu * d;
+        }
+        PC[0 * NC + c] = bnp;   // adjoint of the unfiltered ln p_s^(n+1)
+        bp_c = bcp;
+    }
+    __syncthreads();
+    const double praw = PC[0 * NC + c];
+    if (semi) {
+        // 1. T^(n+1) = T^old + h (R_T - tau DBAR - L_T), P^(n+1) likewise: adjoint of DBAR
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int k = kl + r * AER_SI_KL;
+            if (k >= n) break;
+            const int o = k * NC + c;
+            const double tr = TRAW[o];
+            ot[r] += tr;
+            rt[r] += S.h * tr;
+            double acc = 0.0;
+            for (int kk = k; kk < n; ++kk) acc = acc + S.tau[kk * n + k] * TRAW[kk * NC + c];
+            const double dbar = -S.h * acc - S.h * S.nu[k] * praw;
+            DNRT[o] = draw[r] + S.a1 * dbar;
+            od[r] += (1.0 - S.a1) * dbar;
+            LTB[o] = -S.h * tr;
+        }
+        __syncthreads();
+        // 2. D^(n+1) = Minv RHS: RHS adjoint = Minv^T (adjoint of D^(n+1))
+        const double *Mi = S.Minv + (long)l * n * n;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int k = kl + r * AER_SI_KL;
+            if (k >= n) break;
+            double acc = 0.0;
+            for (int kk = 0; kk < n; ++kk) acc = acc + Mi[kk * n + k] * DNRT[kk * NC + c];
+            RHSB[k * NC + c] = acc;
+        }
+        __syncthreads();
+        // 3. RHS = D^old - hh_old lam W D^old + h (R_D + L_D* - L_D), L_D* = lam (pcoef P* + G T*)
+        if (kl == 0) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc = acc + S.pcoef[k] * RHSB[k * NC + c];
+            PC[1 * NC + c] = lam * S.h * acc;   // adjoint of P*
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int k = kl + r * AER_SI_KL;
+            if (k >= n) break;
+            const int o = k * NC + c;
+            double w = 0.0;
+            for (int kk = 0; kk < n; ++kk) w = w + S.W[kk * n + k] * RHSB[kk * NC + c];
+            od[r] += RHSB[o] - S.hh_old * lam * w;
+            rd[r] += S.h * RHSB[o];
+            LDB[o] = -S.h * RHSB[o];
+            // T* = T^old + astep (R_T - L_T): adjoint of T*
+            double ts = 0.0;
+            for (int kk = 0; kk <= k; ++kk) ts = ts + S.G[kk * n + k] * RHSB[kk * NC + c];
+            ts *= lam * S.h;
+            ot[r] += ts;
+            rt[r] += S.astep * ts;
+            LTB[o] += -S.astep * ts;
+        }
+        __syncthreads();
+        if (kl == 0) {
+            const double bps = PC[1 * NC + c];
+            bp_o += bps + praw;
+            PC[2 * NC + c] = -S.h * praw - S.astep * bps;   // adjoint of L_P
+            PC[1 * NC + c] = S.h * praw + S.astep * bps;    // adjoint of R_P
+        }
+        __syncthreads();
+        // 4. L_D = lam (pcoef P^n + G T^n), L_T = -tau D^n, L_P = -nu . D^n
+        const double blp = PC[2 * NC + c];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int k = kl + r * AER_SI_KL;
+            if (k >= n) break;
+            double tn = 0.0;
+            for (int kk = 0; kk <= k; ++kk) tn = tn + S.G[kk * n + k] * LDB[kk * NC + c];
+            nt[r] += lam * tn;
+            double dn = 0.0;
+            for (int kk = k; kk < n; ++kk) dn = dn + S.tau[kk * n + k] * LTB[kk * NC + c];
+            nd[r] += -dn - S.nu[k] * blp;
+        }
+        if (kl == 0) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc = acc + S.pcoef[k] * LDB[k * NC + c];
+            bp_c += lam * acc;
+        }
+    } else {
+        // explicit leapfrog: D^(n+1) = D^old + h R_D, likewise T and ln p_s
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int k = kl + r * AER_SI_KL;
+            if (k >= n) break;
+            const double tr = TRAW[k * NC + c];
+            od[r] += draw[r];
+            rd[r] += S.h * draw[r];
+            ot[r] += tr;
+            rt[r] += S.h * tr;
+        }
+        if (kl == 0) {
+            bp_o += praw;
+            PC[1 * NC + c] = S.h * praw;
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int k = kl + r * AER_SI_KL;
+        if (k >= n) break;
+        const long x = k * ls + q;
+        S.godiv[x] = od[r];
+        S.gotemp[x] = ot[r];
+        S.gcdiv[x] = nd[r];
+        S.gctemp[x] = nt[r];
+        S.gtdiv[x] = rd[r];
+        S.gttemp[x] = rt[r];
+    }
+    if (kl == 0) {
+        S.golnps[q] = bp_o;
+        S.gclnps[q] = bp_c;
+        S.gtlnps[q] = PC[1 * NC + c];
+    }
+}
+
+}  // namespace aer

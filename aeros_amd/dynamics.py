@@ -29,7 +29,7 @@ class Stepper:
 
     def __init__(self, trunc, a_half, b_half, dt, t_ref=300.0, ps_ref=1.0e5, si_alpha=0.5,
                  tau_diff=0.0, tau_diff_div=0.0, ndiff=6, eps_filter=0.0, raw_alpha=1.0,
-                 held_suarez=True, mass_fixer=False, semi_implicit=True, precision="fp64"):
+                 held_suarez=True, mass_fixer=False, semi_implicit=True, precision="fp64", dteq=None):
         self.sht = AerosSht(trunc, precision=precision)
         self.vg = VGrid(a_half, b_half)
         self.nlev = self.vg.nlev
@@ -42,6 +42,7 @@ class Stepper:
         self.eps, self.raw_alpha = eps_filter, raw_alpha
         self.hs, self.mass_fixer, self.semi = held_suarez, mass_fixer, semi_implicit
         self.phis = None
+        self.dteq = dteq          # optional additive Held-Suarez T_eq [nlev, nlat] (CUDA float64)
         self.nstep = 0
         self.mass_target = None
         self.lnr_cum = None       # running sum of the mass fixer's ln r (device scalar)
@@ -67,7 +68,7 @@ class Stepper:
             self.si.set_step(h)
             self.h = h
         g = s.tendency_grid_transforms(now["vor"], now["div"], now["temp"], now["lnps"])
-        c = column_terms(s, self.vg, g, phis=self.phis, held_suarez=self.hs)
+        c = column_terms(s, self.vg, g, phis=self.phis, held_suarez=self.hs, dteq=self.dteq)
         tnd = s.tendency_spectral(c["ae"], c["an"], c["ek"], c["dtdt"], c["dlnpsdt"])
         eps = self.eps if self.nstep > 0 else 0.0
         new = self.si.leapfrog(self.old, now, tnd, semi_implicit=self.semi, diffuse=self.diffuse,

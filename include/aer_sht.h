@@ -282,6 +282,30 @@ int aer_column_terms_d(aer_sht_t cfg, aer_vgrid_t vg, int held_suarez,
                        const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
                        const double *phis,
                        double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt);
+/* The same with the Held-Suarez forcing on and an additive equilibrium-temperature
+ * field dteq [nlev][nlat] (T_eq = formula + dteq): the control variable of the adjoint.
+ * No reference counterpart (the reference's T_eq is the formula alone). */
+int aer_column_terms_dteq_d(aer_sht_t cfg, aer_vgrid_t vg, const double *dteq,
+                            const double *u, const double *v, const double *vor_g, const double *div_g,
+                            const double *t_g, const double *dtdx, const double *dtdy,
+                            const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                            const double *phis,
+                            double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt);
+/* Adjoint (transpose of the linearization at the given inputs) of aer_column_terms_d /
+ * aer_column_terms_dteq_d: from the adjoints of (ae, an, ek, dtdt, dlnpsdt) to those of the
+ * ten grid inputs (overwritten) and, if g_teq is not null, the per-column sensitivity to
+ * dteq [nlev][nlat*nlon] (sum it over longitudes for the field's gradient). nlev <= 64.
+ * No reference counterpart. */
+int aer_column_terms_adj_d(aer_sht_t cfg, aer_vgrid_t vg, int held_suarez, const double *dteq,
+                           const double *u, const double *v, const double *vor_g, const double *div_g,
+                           const double *t_g, const double *dtdx, const double *dtdy,
+                           const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                           const double *phis,
+                           const double *b_ae, const double *b_an, const double *b_ek, const double *b_dtdt,
+                           const double *b_dlnpsdt,
+                           double *g_u, double *g_v, double *g_vor, double *g_div, double *g_t,
+                           double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx, double *g_dlnpsdy,
+                           double *g_teq);
 
 /* ---- spectral step completion: the semi-implicit gravity-wave solve
  * (src/dynamics/aeros_semiimp.f90) and the leapfrog pieces of
@@ -331,6 +355,18 @@ int aer_leapfrog_d(aer_semiimp_t si, int flags, double eps_filter, double raw_al
                    double *now_vor, double *now_div, double *now_temp, double *now_lnps,
                    const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
                    double *new_vor, double *new_div, double *new_temp, double *new_lnps);
+
+/* Adjoint of aer_leapfrog_d with the same flags, filter and step settings: the transpose
+ * of the linear map (X^(n-1), X^n, R) -> (X^(n+1), filtered X^n). Inputs are the adjoints
+ * of X^(n+1) (bn_*) and of the filtered X^n (bc_*); outputs (overwritten) are the
+ * adjoints of X^(n-1) (go_*), of X^n as it entered the step (gc_*) and of the tendency
+ * (gt_*). No reference counterpart. */
+int aer_leapfrog_adj_d(aer_semiimp_t si, int flags, double eps_filter, double raw_alpha,
+                       const double *bn_vor, const double *bn_div, const double *bn_temp, const double *bn_lnps,
+                       const double *bc_vor, const double *bc_div, const double *bc_temp, const double *bc_lnps,
+                       double *go_vor, double *go_div, double *go_temp, double *go_lnps,
+                       double *gc_vor, double *gc_div, double *gc_temp, double *gc_lnps,
+                       double *gt_vor, double *gt_div, double *gt_temp, double *gt_lnps);
 
 /* Global dry-air mass of a spectral ln p_s (mass_of, src/dynamics/aeros_timestep.f90:1250-1276):
  * one FP64 synthesis and an FP64 quadrature, written to the device double

@@ -234,3 +234,47 @@ class HeldSuarezAdjoint:
         if st.mass_fixer:
             lam_now["lnps"] = self.mass_target_adjoint(lam_now["lnps"], lnps0)
         return J, lam_now, g_teq
+
+
+def seeded_directions(st, n, config=3, zeta_rms=1.0e-5, temp_rms=10.0, teq_rms=10.0):
+    """n reproducible directions in (initial zeta, initial T, dteq): seeded red spectra
+    (seed 1000 config + field + 100000 direction) scaled to the given grid rms, and a
+    seeded normal dteq field; divergence and ln p_s directions are zero."""
+    import numpy as np
+
+    from .seeding import normals, red_spectra
+    s = st.sht
+    out = []
+    for i in range(n):
+        dev = lambda a: torch.as_tensor(a, dtype=torch.complex128).cuda().contiguous()
+        z = dev(red_spectra(s.trunc, st.nlev, config, member=i, field0=0, zero_l0=True))
+        t = dev(red_spectra(s.trunc, st.nlev, config, member=i, field0=100, zero_l0=True))
+        z *= zeta_rms / float(s.synthesis(z).pow(2).mean().sqrt())
+        t *= temp_rms / float(s.synthesis(t).pow(2).mean().sqrt())
+        q = normals(1000 * config + 200 + 100000 * i, st.nlev * s.nlat).real.reshape(st.nlev, s.nlat)
+        d = {"vor": z, "div": torch.zeros_like(z), "temp": t,
+             "lnps": torch.zeros(s.nlm, dtype=torch.complex128, device="cuda")}
+        out.append((d, torch.as_tensor(np.ascontiguousarray(q) * teq_rms, dtype=torch.float64).cuda()))
+    return out
+
+
+def directional_check(adj, state, nsteps, directions, eps=1.0e-6, every=12):
+    """Central differences (J(x + eps d) - J(x - eps d)) / (2 eps) of the forward code along
+    each (state direction, dteq direction) against the adjoint's <grad, d>. Returns J, the
+    gradient and one (fd, adjoint, relative difference) per direction."""
+    st = adj.st
+    q0 = st.dteq if st.dteq is not None else torch.zeros((st.nlev, st.sht.nlat), dtype=torch.float64,
+                                                          device="cuda")
+    st.dteq = q0
+    J, lam, gq = adj.gradient(state, nsteps, every=every)
+    res = []
+    for d, dq in directions:
+        vals = []
+        for sgn in (1.0, -1.0):
+            st.dteq = (q0 + sgn * eps * dq).contiguous()
+            vals.append(adj.run([x + sgn * eps * d[k] for x, k in zip(state, KEYS)], nsteps))
+        st.dteq = q0
+        fd = (vals[0] - vals[1]) / (2.0 * eps)
+        ad = dot(lam, d) + dot(gq, dq)
+        res.append((fd, ad, abs(fd - ad) / max(abs(ad), 1e-300)))
+    return J, lam, gq, res

@@ -456,6 +456,78 @@ def held_suarez_climate(torch, trunc=31, days=1200.0):
     return out
 
 
+def adjoint_T31(torch, spinup_days=100.0, nsteps=96, every=12, ndir=16, eps=1e-6):
+    """Gradient of the Held-Suarez functional (mass-weighted mean T over 30-60 N at the final
+    step) through 96 T31L20 steps in FP64, checkpoints every 12, from a spun-up flow:
+    timing of the forward run and of the checkpointed gradient, peak device memory, the
+    dot-product residual of each transpose transform at T31, and central differences along
+    16 seeded directions in (initial zeta, initial T, 2-D T_eq)."""
+    from aeros_amd.adjoint import HeldSuarezAdjoint, directional_check, seeded_directions
+    from aeros_amd.column import vgrid_init
+    from aeros_amd.dynamics import Stepper
+    from aeros_amd.held_suarez import initial_state
+    from aeros_amd.seeding import red_spectra
+    nlev, trunc, dt = 20, 31, 1800.0
+    A, B = vgrid_init(nlev, stretch_a=1.0, stretch_r=2.0, sigma_t=0.0, p_top=1000.0)
+    st = Stepper(trunc, A, B, dt, tau_diff=6 * 3600.0, ndiff=6, eps_filter=0.06, raw_alpha=0.53,
+                 held_suarez=True, mass_fixer=True)
+    s = st.sht
+    st.set_state(*initial_state(s, nlev))
+    for _ in range(int(spinup_days * 86400.0 / dt)):
+        st.step()
+    x0 = [st.now[k].clone() for k in ("vor", "div", "temp", "lnps")]
+    st.dteq = torch.zeros((nlev, s.nlat), dtype=torch.float64, device="cuda")
+    adj = HeldSuarezAdjoint(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    adj.run(x0, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    J, lam, gq = adj.gradient(x0, nsteps, every=every)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    # dot-product residuals of the transpose transforms at this truncation
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    c1, c2 = dev(red_spectra(trunc, nlev, 3, field0=500)), dev(red_spectra(trunc, nlev, 3, field0=600))
+    g1 = s.synthesis(dev(red_spectra(trunc, nlev, 3, field0=700)))
+    g2 = s.synthesis(dev(red_spectra(trunc, nlev, 3, field0=800)))
+    sd = lambda a, b: float((torch.view_as_real(a) * torch.view_as_real(b)).sum())
+    gd = lambda a, b: float((a * b).sum())
+    res = {}
+    x = s.synthesis(c1)
+    res["synthesis"] = abs(gd(x, g1) - sd(c1, s.synthesis_adjoint(g1))) / abs(gd(x, g1))
+    x = s.analysis(g1)
+    res["analysis"] = abs(sd(x, c1) - gd(g1, s.analysis_adjoint(c1))) / abs(sd(x, c1))
+    vt, vp = s.synthesis_vec(c1, c2)
+    a_, b_ = s.synthesis_vec_adjoint(g1, g2)
+    lhs = gd(vt, g1) + gd(vp, g2)
+    res["synthesis_vec"] = abs(lhs - sd(c1, a_) - sd(c2, b_)) / abs(lhs)
+    sa, ta = s.analysis_vec(g1, g2)
+    at, ap = s.analysis_vec_adjoint(c1, c2)
+    lhs = sd(sa, c1) + sd(ta, c2)
+    res["analysis_vec"] = abs(lhs - gd(g1, at) - gd(g2, ap)) / abs(lhs)
+    gt, gp = s.gradient(c1)
+    lhs = gd(gt, g1) + gd(gp, g2)
+    res["gradient"] = abs(lhs - sd(c1, s.gradient_adjoint(g1, g2))) / abs(lhs)
+    _, _, _, chk = directional_check(adj, x0, nsteps, seeded_directions(st, ndir), eps=eps, every=every)
+    rels = sorted(r for _, _, r in chk)
+    alg = nsteps * 3 * sref_bytes(s.nlon, s.nlat, s.nlm, nlev)
+    return {"workload": f"T{trunc}L{nlev} Held-Suarez, {nsteps} steps of {dt:.0f} s from day {spinup_days:.0f}, "
+                        f"FP64, checkpoints every {every}",
+            "objective": "mass-weighted mean T over 30-60 N at the final step", "J_K": round(J, 9),
+            "forward_ms_per_step": round(1000 * t_fwd / nsteps, 4),
+            "gradient_s": round(t_grad, 4),
+            "gradient_over_forward": round(t_grad / t_fwd, 2),
+            "peak_device_memory_MB": round(peak / 1e6, 1),
+            "alg_bytes_whole_run": int(alg), "alg_GBs_whole_run": round(alg / t_grad / 1e9, 1),
+            "dot_product_rel_residual": {k: float(f"{v:.2e}") for k, v in res.items()},
+            "fd_directions": ndir, "fd_eps": eps,
+            "fd_rel_err_max": float(f"{rels[-1]:.3e}"), "fd_rel_err_median": float(f"{rels[len(rels) // 2]:.3e}")}
+
+
 def dynamics_step(torch, precision, steps=20, warmup=3, path="staged"):
     """Whole Held-Suarez dynamics steps at T42L20 (transform halves, column terms +
     forcing, semi-implicit leapfrog with diffusion and filter, mass fixer), eager
@@ -509,7 +581,7 @@ def main():
     ap.add_argument("--no-hs", action="store_true", help="skip the 1200-day Held-Suarez climate runs (~25 s)")
     ap.add_argument("--no-stress", action="store_true", help="skip the 32 x T106L30 stress plan")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--only", choices=["set", "ensemble", "step", "stress", "hs"], default=None,
+    ap.add_argument("--only", choices=["set", "ensemble", "step", "stress", "hs", "adjoint"], default=None,
                     help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms), only the "
                          "ensemble, or only whole Held-Suarez dynamics steps")
     ap.add_argument("--path", choices=["auto", "tiled", "staged"], default="auto",
@@ -544,6 +616,9 @@ def main():
         return
     if args.only == "hs":
         print(json.dumps({"held_suarez_T31L20_1200d": held_suarez_climate(torch)}))
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint_T31L20_96": adjoint_T31(torch)}))
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup,
@@ -688,6 +763,7 @@ def main():
                 "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
         "ms_per_step_fp64": dynamics_step(torch, "fp64", path=args.path),
         "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path)}
+    line["adjoint_T31L20_96"] = adjoint_T31(torch)
     if not args.no_hs:
         line["held_suarez_T31L20_1200d"] = held_suarez_climate(torch)
     if not args.no_stress:

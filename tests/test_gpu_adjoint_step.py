@@ -198,3 +198,36 @@ def test_gradient_independent_of_checkpoint_spacing(torch):
     for k in SKEYS:
         assert torch.equal(la[k], lb[k]), k
     assert torch.equal(qa, qb)
+
+
+def hs_T31L20(torch, spinup_days):
+    """The Held-Suarez protocol at T31L20 (uniform sigma, 1000 Pa top, dt 1800 s, del^6 with
+    a 6 h e-folding, RAW filter, mass fixer), spun up from the seeded isothermal start."""
+    from aeros_amd.column import vgrid_init
+    from aeros_amd.dynamics import Stepper
+    from aeros_amd.held_suarez import initial_state
+    A, B = vgrid_init(20, stretch_a=1.0, stretch_r=2.0, sigma_t=0.0, p_top=1000.0)
+    st = Stepper(31, A, B, 1800.0, tau_diff=6 * 3600.0, ndiff=6, eps_filter=0.06, raw_alpha=0.53,
+                 held_suarez=True, mass_fixer=True)
+    st.set_state(*initial_state(st.sht, 20))
+    for _ in range(int(spinup_days * 48)):
+        st.step()
+    x0 = [st.now[k].clone() for k in SKEYS]
+    st.dteq = torch.zeros((20, st.sht.nlat), dtype=torch.float64, device="cuda")
+    return st, x0
+
+
+def test_config3_gradient_T31L20_96_steps(torch):
+    """96 steps at T31L20 from a developed (60-day) Held-Suarez flow, checkpoints every 12:
+    the gradient w.r.t. the initial zeta, T and the 2-D T_eq field against central
+    differences with eps = 1e-6 along 16 seeded directions."""
+    from aeros_amd.adjoint import HeldSuarezAdjoint, directional_check, seeded_directions
+    st, x0 = hs_T31L20(torch, 60)
+    adj = HeldSuarezAdjoint(st)
+    dirs = seeded_directions(st, 16)
+    J, lam, gq, res = directional_check(adj, x0, 96, dirs, eps=1e-6, every=12)
+    rels = [r for _, _, r in res]
+    for i, (fd, ad, r) in enumerate(res):
+        print(f"direction {i:2d}: fd {fd:+.10e} adjoint {ad:+.10e} rel {r:.2e}", flush=True)
+    print(f"J {J:.12f}  max rel {max(rels):.2e}  median {sorted(rels)[8]:.2e}", flush=True)
+    assert max(rels) <= 1e-5

@@ -113,6 +113,13 @@ SIGNATURES["aer_leapfrog_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_
 SIGNATURES["aer_mass_d"] = (_c_int, [_vp, _vp, _c_int, _c_double, _vp, _vp])
 SIGNATURES["aer_leapfrog_adj_d"] = (_c_int, [_vp, _c_int, _c_double, _c_double] + [_vp] * 20)
 AER_STEP_SEMI_IMPLICIT, AER_STEP_DIFFUSE, AER_STEP_FILTER = 1, 2, 4
+SIGNATURES["aer_tendency_grid_transforms_batch_d"] = (_c_int, [_vp, _c_int, _c_int] + [_vp] * 14)
+SIGNATURES["aer_tendency_spectral_batch_d"] = (_c_int, [_vp, _c_int, _c_int] + [_vp] * 9)
+SIGNATURES["aer_column_terms_batch_d"] = (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long] + [_vp] * 16)
+SIGNATURES["aer_column_terms_adj_batch_d"] = (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long] + [_vp] * 27)
+SIGNATURES["aer_leapfrog_batch_d"] = (_c_int, [_vp, _c_int, _c_int, _c_double, _c_double] + [_vp] * 16)
+SIGNATURES["aer_leapfrog_adj_batch_d"] = (_c_int, [_vp, _c_int, _c_int, _c_double, _c_double] + [_vp] * 20)
+SIGNATURES["aer_mass_batch_d"] = (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp])
 SIGNATURES["aer_diag_create"] = (_c_int, [_vp, _c_int, ctypes.POINTER(_vp)])
 SIGNATURES["aer_diag_accumulate_d"] = (_c_int, [_vp, _vp, _vp, _vp, _vp])
 SIGNATURES["aer_diag_read"] = (_c_int, [_vp, _ip, _dp])

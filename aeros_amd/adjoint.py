@@ -24,8 +24,6 @@ import math
 
 import torch
 
-from . import _lib
-from ._lib import check, lib
 from .column import column_terms, column_terms_adjoint
 from .step import mass
 
@@ -52,8 +50,8 @@ class HeldSuarezAdjoint:
     """Forward run with checkpoints, and its reverse sweep, for one Stepper configuration."""
 
     def __init__(self, stepper):
-        if stepper.sht.precision != "fp64":
-            raise ValueError("the adjoint is the exact transpose of the FP64 transforms only")
+        if stepper.sht.precision not in ("fp64", "split"):
+            raise ValueError("the adjoint transposes the FP64 or split-FP32 transforms (near-exact) only")
         self.st = stepper
         self.target_bar = 0.0
         s = stepper.sht
@@ -66,7 +64,10 @@ class HeldSuarezAdjoint:
         self.f_sraw_of_tdiv = -ll1 / RADIUS        # tdiv = this * S (S = anal_vec's 1/(l(l+1)) form)
         self.f_traw_of_tvor = ll1 / RADIUS
         self.f_neg_lap = ll1 / (RADIUS * RADIUS)
+        prec = s.precision
+        s.set_precision("fp64")   # c00 as the mass fixer reads it (an FP64 analysis of 1)
         self.c00 = float(s.analysis(torch.ones((s.nlat, s.nlon), dtype=torch.float64, device="cuda"))[0].real)
+        s.set_precision(prec)
         w = torch.as_tensor(s.gauss_w, dtype=torch.float64, device="cuda")
         self.wrow = (w * (2.0 * math.pi / s.nlon))[:, None]
 
@@ -80,9 +81,12 @@ class HeldSuarezAdjoint:
         w = torch.as_tensor(s.gauss_w, dtype=torch.float64, device="cuda")
         wk = dp[:, None] * (w * band)[None, :]                  # [nlev, nlat]
         wk = wk / (wk.sum() * s.nlon)
-        return wk[:, :, None].expand(self.st.nlev, s.nlat, s.nlon).contiguous()
+        w = wk[:, :, None].expand(self.st.nlev, s.nlat, s.nlon)
+        shape = tuple(self.st.now["temp"].shape[:-1]) + (s.nlat, s.nlon)
+        return w.expand(shape).contiguous()
 
     def objective(self, now):
+        """J (for an ensemble, the sum of the members' values)."""
         t = self.st.sht.synthesis(now["temp"])
         return float((t * self.objective_weights()).sum())
 
@@ -104,7 +108,7 @@ class HeldSuarezAdjoint:
             h = st.dt
             st.old = {k: v.clone() for k, v in now.items()}
             if st.mass_fixer and st.mass_target is None:
-                st.mass_target = mass(s, now["lnps"])[0].item()
+                st.mass_target = st._mass_target(now["lnps"])
         else:
             h = 2.0 * st.dt
         if h != st.h:
@@ -149,17 +153,17 @@ class HeldSuarezAdjoint:
     def mass_fix_adjoint(self, lam_lnps, lnps_prefix):
         """Transpose of ln p_s -> ln p_s + c00 ln(target / M(ln p_s)) e00 at ln p_s = lnps_prefix."""
         s = self.st.sht
-        eg = torch.exp(s.synthesis(lnps_prefix)) * self.wrow
-        q = float(eg.sum())
-        coef = self.c00 * float(lam_lnps[0].real)
-        self.target_bar += coef      # d(ln target) sensitivity, added to the initial ln p_s at the end
-        return lam_lnps - (coef / q) * s.synthesis_adjoint(eg)
+        eg = torch.exp(s.synthesis(lnps_prefix)) * self.wrow       # ([nmem,] nlat, nlon)
+        q = eg.sum(dim=(-2, -1))
+        coef = self.c00 * lam_lnps[..., 0].real
+        self.target_bar = self.target_bar + coef   # d(ln target), added to the initial ln p_s at the end
+        return lam_lnps - (coef / q)[..., None] * s.synthesis_adjoint(eg)
 
     def mass_target_adjoint(self, lam_lnps, lnps0):
         """Adds the target's share: ln target = ln M(ln p_s at step 0)."""
         s = self.st.sht
         eg = torch.exp(s.synthesis(lnps0)) * self.wrow
-        return lam_lnps + (self.target_bar / float(eg.sum())) * s.synthesis_adjoint(eg)
+        return lam_lnps + (self.target_bar / eg.sum(dim=(-2, -1)))[..., None] * s.synthesis_adjoint(eg)
 
     def step_adjoint(self, rec, lam_old, lam_now):
         """(adjoint of X^n filtered, adjoint of X^(n+1) after the fixer) -> (adjoints of
@@ -172,15 +176,9 @@ class HeldSuarezAdjoint:
         if rec["h"] != st.h:
             st.si.set_step(rec["h"])
             st.h = rec["h"]
-        flags = ((_lib.AER_STEP_SEMI_IMPLICIT if st.semi else 0) | (_lib.AER_STEP_DIFFUSE if st.diffuse else 0) |
-                 (_lib.AER_STEP_FILTER if rec["eps"] > 0 else 0))
-        go = {k: torch.empty_like(v) for k, v in lam_new.items()}
-        gc = {k: torch.empty_like(v) for k, v in lam_new.items()}
-        gt = {k: torch.empty_like(v) for k, v in lam_new.items()}
-        P = lambda d: [d[k].contiguous().data_ptr() for k in KEYS]
-        s._bind_stream()
-        check(lib().aer_leapfrog_adj_d(st.si._h, flags, rec["eps"], st.raw_alpha, *P(lam_new), *P(lam_old),
-                                       *P(go), *P(gc), *P(gt)))
+        c = lambda d: {k: d[k].contiguous() for k in KEYS}
+        go, gc, gt = st.si.leapfrog_adjoint(c(lam_new), c(lam_old), semi_implicit=st.semi, diffuse=st.diffuse,
+                                            eps_filter=rec["eps"], raw_alpha=st.raw_alpha)
         cbar = self.ts_adjoint(gt)
         now = rec["now"]
         g = s.tendency_grid_transforms(now["vor"], now["div"], now["temp"], now["lnps"])
@@ -213,7 +211,7 @@ class HeldSuarezAdjoint:
         st.mass_target = None
         st.lnr_cum = None
         lnps0 = st.now["lnps"].clone()
-        self.target_bar = 0.0
+        self.target_bar = torch.zeros(lnps0.shape[:-1], dtype=torch.float64, device="cuda")
         ckpts = {0: st.checkpoint()}
         for n in range(nsteps):
             st.step()
@@ -222,7 +220,7 @@ class HeldSuarezAdjoint:
         J = self.objective(st.now)
         lam_old = {k: torch.zeros_like(v) for k, v in st.now.items()}
         lam_now = self.objective_adjoint()
-        g_teq = torch.zeros((st.nlev, st.sht.nlat), dtype=torch.float64, device="cuda")
+        g_teq = None
         for start in sorted(ckpts, reverse=True):
             end = min(start + every, nsteps)
             st.restore(ckpts[start])
@@ -230,10 +228,16 @@ class HeldSuarezAdjoint:
             for rec in reversed(recs):
                 lam_old, lam_now, teq = self.step_adjoint(rec, lam_old, lam_now)
                 if teq is not None:
-                    g_teq += teq
+                    g_teq = teq if g_teq is None else g_teq + teq
         if st.mass_fixer:
             lam_now["lnps"] = self.mass_target_adjoint(lam_now["lnps"], lnps0)
+        if g_teq is None:
+            g_teq = torch.zeros(self.teq_shape(), dtype=torch.float64, device="cuda")
         return J, lam_now, g_teq
+
+    def teq_shape(self):
+        """Shape of a per-member dteq field: (nlev, nlat), or (nmem, nlev, nlat)."""
+        return tuple(self.st.now["temp"].shape[:-1]) + (self.st.sht.nlat,)
 
 
 def seeded_directions(st, n, config=3, zeta_rms=1.0e-5, temp_rms=10.0, teq_rms=10.0):
@@ -263,8 +267,8 @@ def directional_check(adj, state, nsteps, directions, eps=1.0e-6, every=12):
     each (state direction, dteq direction) against the adjoint's <grad, d>. Returns J, the
     gradient and one (fd, adjoint, relative difference) per direction."""
     st = adj.st
-    q0 = st.dteq if st.dteq is not None else torch.zeros((st.nlev, st.sht.nlat), dtype=torch.float64,
-                                                          device="cuda")
+    st.set_state(*state)
+    q0 = st.dteq if st.dteq is not None else torch.zeros(adj.teq_shape(), dtype=torch.float64, device="cuda")
     st.dteq = q0
     J, lam, gq = adj.gradient(state, nsteps, every=every)
     res = []

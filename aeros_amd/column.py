@@ -65,55 +65,81 @@ class VGrid:
             self._h = None
 
 
+GKEYS = ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy", "lnps_g", "dlnpsdx", "dlnpsdy")
+CKEYS = ("ae", "an", "ek", "dtdt", "dlnpsdt")
+
+
+def _members(sht, vg, grid, phis, dteq, held_suarez):
+    """(nmem, dteq pointer, dteq member stride) after checking the grid dict's shapes:
+    3-D fields ([nmem,] nlev, nlat, nlon), 2-D ([nmem,] nlat, nlon); dteq (nlev, nlat)
+    shared by every member or (nmem, nlev, nlat)."""
+    import torch
+    u = grid["u"]
+    lead = tuple(u.shape[:-3])
+    if len(lead) > 1 or u.shape[-3:] != (vg.nlev, sht.nlat, sht.nlon):
+        raise ValueError("grid fields do not match the vertical grid / transform grid")
+    for k in GKEYS:
+        t = grid[k]
+        want = u.shape if k in GKEYS[:7] else lead + (sht.nlat, sht.nlon)
+        if tuple(t.shape) != tuple(want):
+            raise ValueError(f"{k}: expected shape {tuple(want)}")
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
+    if phis is not None and not (phis.is_cuda and phis.dtype == torch.float64 and phis.is_contiguous()
+                                 and tuple(phis.shape) == (sht.nlat, sht.nlon)):
+        raise ValueError("phis: expected a contiguous float64 CUDA tensor (nlat, nlon)")
+    nmem = lead[0] if lead else 1
+    if dteq is None:
+        return nmem, None, 0
+    # Held-Suarez with T_eq = formula + dteq (the adjoint's control variable)
+    if not held_suarez:
+        raise ValueError("dteq needs the Held-Suarez forcing")
+    if not (dteq.is_cuda and dteq.dtype == torch.float64 and dteq.is_contiguous()):
+        raise ValueError("dteq: expected a contiguous float64 CUDA tensor")
+    if tuple(dteq.shape) == (vg.nlev, sht.nlat):
+        return nmem, dteq.data_ptr(), 0
+    if lead and tuple(dteq.shape) == (nmem, vg.nlev, sht.nlat):
+        return nmem, dteq.data_ptr(), vg.nlev * sht.nlat
+    raise ValueError("dteq: expected shape (nlev, nlat) or (nmem, nlev, nlat)")
+
+
 def column_terms(sht, vg, grid, phis=None, held_suarez=False, out=None, dteq=None):
     """Grid right-hand sides of every column from the dict that
     AerosSht.tendency_grid_transforms returns. Returns ae, an, ek, dtdt (3-D) and
-    dlnpsdt (2-D) as a dict; inputs are left unchanged."""
+    dlnpsdt (2-D) as a dict; inputs are left unchanged. An ensemble passes fields with a
+    leading member dimension (one launch for all members)."""
     import torch
+    nmem, dq, dqs = _members(sht, vg, grid, phis, dteq, held_suarez)
     u = grid["u"]
-    nlev = u.shape[0]
-    if nlev != vg.nlev or u.shape[1:] != (sht.nlat, sht.nlon):
-        raise ValueError("grid fields do not match the vertical grid / transform grid")
-    for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy", "lnps_g", "dlnpsdx", "dlnpsdy"):
-        t = grid[k]
-        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
-            raise ValueError(f"{k}: expected a contiguous float64 CUDA tensor")
-    if phis is not None and not (phis.is_cuda and phis.dtype == torch.float64 and phis.is_contiguous()):
-        raise ValueError("phis: expected a contiguous float64 CUDA tensor")
     if out is None:
         out = {k: torch.empty_like(u) for k in ("ae", "an", "ek", "dtdt")}
         out["dlnpsdt"] = torch.empty_like(grid["lnps_g"])
     sht._bind_stream()
-    ins = [grid[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
-                                         "lnps_g", "dlnpsdx", "dlnpsdy")]
-    outs = [out[k].data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")]
-    ph = phis.data_ptr() if phis is not None else None
-    if dteq is not None:
-        # Held-Suarez with T_eq = formula + dteq ([nlev, nlat]; the adjoint's control variable)
-        if not held_suarez:
-            raise ValueError("dteq needs the Held-Suarez forcing")
-        if tuple(dteq.shape) != (nlev, sht.nlat) or not (dteq.is_cuda and dteq.dtype == torch.float64):
-            raise ValueError("dteq: expected a float64 CUDA tensor of shape (nlev, nlat)")
-        check(lib().aer_column_terms_dteq_d(sht._h, vg._h, dteq.contiguous().data_ptr(), *ins, ph, *outs))
-    else:
-        check(lib().aer_column_terms_d(sht._h, vg._h, 1 if held_suarez else 0, *ins, ph, *outs))
+    check(lib().aer_column_terms_batch_d(
+        sht._h, vg._h, nmem, 1 if held_suarez else 0, dq, dqs, *[grid[k].data_ptr() for k in GKEYS],
+        phis.data_ptr() if phis is not None else None, *[out[k].data_ptr() for k in CKEYS]))
     return out
 
 
 def column_terms_adjoint(sht, vg, grid, cbar, phis=None, held_suarez=False, dteq=None, want_teq=False):
     """Adjoint of column_terms at the inputs `grid`: from the adjoints of ae, an, ek, dtdt,
     dlnpsdt (dict cbar) to those of the ten grid inputs, and, with want_teq, the sensitivity
-    to dteq per (level, latitude) (summed over longitudes)."""
+    to dteq per (level, latitude) (summed over longitudes; for an ensemble per member,
+    or summed over members when dteq is one shared field)."""
     import torch
-    keys = ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy", "lnps_g", "dlnpsdx", "dlnpsdy")
-    g = {k: torch.empty_like(grid[k]) for k in keys}
+    nmem, dq, dqs = _members(sht, vg, grid, phis, dteq, held_suarez)
+    g = {k: torch.empty_like(grid[k]) for k in GKEYS}
+    cb = {k: cbar[k].contiguous() for k in CKEYS}
     gteq = torch.empty_like(grid["u"]) if want_teq else None
     sht._bind_stream()
-    check(lib().aer_column_terms_adj_d(
-        sht._h, vg._h, 1 if held_suarez else 0, dteq.contiguous().data_ptr() if dteq is not None else None,
-        *[grid[k].data_ptr() for k in keys], phis.data_ptr() if phis is not None else None,
-        *[cbar[k].contiguous().data_ptr() for k in ("ae", "an", "ek", "dtdt", "dlnpsdt")],
-        *[g[k].data_ptr() for k in keys], gteq.data_ptr() if gteq is not None else None))
-    if want_teq:
-        return g, gteq.sum(dim=2)
-    return g
+    check(lib().aer_column_terms_adj_batch_d(
+        sht._h, vg._h, nmem, 1 if held_suarez else 0, dq, dqs, *[grid[k].data_ptr() for k in GKEYS],
+        phis.data_ptr() if phis is not None else None,
+        *[cb[k].data_ptr() for k in CKEYS],
+        *[g[k].data_ptr() for k in GKEYS], gteq.data_ptr() if gteq is not None else None))
+    if not want_teq:
+        return g
+    gq = gteq.sum(dim=-1)
+    if dteq is not None and dteq.dim() == 2 and gq.dim() == 3:
+        gq = gq.sum(dim=0)
+    return g, gq

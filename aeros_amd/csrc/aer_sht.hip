@@ -1602,12 +1602,24 @@ int aer_tendency_grid_transforms_d(aer_sht_t h, int nlev,
         double *u, double *v, double *vor_g, double *div_g, double *t_g,
         double *dtdx, double *dtdy, double *lnps_g, double *dlnpsdx, double *dlnpsdy)
 {
-    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    return aer_tendency_grid_transforms_batch_d(h, 1, nlev, vor, div, temp, lnps, u, v, vor_g, div_g, t_g,
+                                                dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy);
+}
+
+int aer_tendency_grid_transforms_batch_d(aer_sht_t h, int nmem, int nlev,
+        const double *vor, const double *div, const double *temp, const double *lnps,
+        double *u, double *v, double *vor_g, double *div_g, double *t_g,
+        double *dtdx, double *dtdy, double *lnps_g, double *dlnpsdx, double *dlnpsdy)
+{
+    if (nmem < 1 || nlev < 1 || (long)nmem * nlev > (1L << 30))
+        return fail(AER_EINVAL, "aer_tendency_grid_transforms_batch_d: bad member or level count");
+    int rc = check_batch(h, nmem * nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
     if (rc) return rc;
     const long ss = dense_s(h), gs = dense_g(h);
-    JobSpec j[7] = {{aer::SYN_SCALAR, 1}, {aer::SYN_GRAD, 1}, {aer::SYN_VECTOR, nlev},
-                    {aer::SYN_SCALAR, nlev}, {aer::SYN_SCALAR, nlev}, {aer::SYN_SCALAR, nlev},
-                    {aer::SYN_GRAD, nlev}};
+    const int nf = nmem * nlev;
+    JobSpec j[7] = {{aer::SYN_SCALAR, nmem}, {aer::SYN_GRAD, nmem}, {aer::SYN_VECTOR, nf},
+                    {aer::SYN_SCALAR, nf}, {aer::SYN_SCALAR, nf}, {aer::SYN_SCALAR, nf},
+                    {aer::SYN_GRAD, nf}};
     for (auto &s : j) { s.sstride = ss; s.gstride = gs; }
     j[0].sin0 = lnps; j[0].gout0 = lnps_g;
     j[1].sin0 = lnps; j[1].gout0 = dlnpsdy; j[1].gsc0 = -1.0 / kREarth; j[1].gout1 = dlnpsdx; j[1].gsc1 = 1.0 / kREarth;
@@ -1624,10 +1636,20 @@ int aer_tendency_spectral_d(aer_sht_t h, int nlev,
         const double *ae, const double *an, const double *ek, const double *dtdt,
         const double *dlnpsdt, double *tvor, double *tdiv, double *ttemp, double *tlnps)
 {
-    int rc = check_batch(h, nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
+    return aer_tendency_spectral_batch_d(h, 1, nlev, ae, an, ek, dtdt, dlnpsdt, tvor, tdiv, ttemp, tlnps);
+}
+
+int aer_tendency_spectral_batch_d(aer_sht_t h, int nmem, int nlev,
+        const double *ae, const double *an, const double *ek, const double *dtdt,
+        const double *dlnpsdt, double *tvor, double *tdiv, double *ttemp, double *tlnps)
+{
+    if (nmem < 1 || nlev < 1 || (long)nmem * nlev > (1L << 30))
+        return fail(AER_EINVAL, "aer_tendency_spectral_batch_d: bad member or level count");
+    int rc = check_batch(h, nmem * nlev, h ? dense_s(h) : 0, h ? dense_g(h) : 0);
     if (rc) return rc;
     const long ss = dense_s(h), gs = dense_g(h);
-    JobSpec j[3] = {{aer::ANA_VECTOR, nlev}, {aer::ANA_SCALAR, nlev}, {aer::ANA_SCALAR, 1}};
+    const int nf = nmem * nlev;
+    JobSpec j[3] = {{aer::ANA_VECTOR, nf}, {aer::ANA_SCALAR, nf}, {aer::ANA_SCALAR, nmem}};
     for (auto &s : j) { s.sstride = ss; s.gstride = gs; }
     j[0].gin0 = an; j[0].gsc0 = -1.0; j[0].gin1 = ae; j[0].gin2 = ek;
     j[0].sout0 = tdiv; j[0].mode0 = aer::M_DIV_FROM_S;
@@ -1791,14 +1813,26 @@ int column_args(aer_sht *h, aer_vgrid *vg, int held_suarez, const double *u, con
     return AER_OK;
 }
 
-int column_launch(aer_sht *h, aer::ColumnArgs &C, double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt,
-                  const char *who)
+// Member strides of ensemble arrays: [nmem][nlev][ncol], [nmem][ncol], dteq [nmem][nlev][nlat]
+// (or shared when dteq_mstride is 0).
+int column_members(aer::ColumnArgs &C, int nmem, long dteq_mstride, const char *who)
+{
+    if (nmem < 1 || nmem > 65535) return fail(AER_EINVAL, std::string(who) + ": member count outside 1..65535");
+    if (dteq_mstride < 0) return fail(AER_EINVAL, std::string(who) + ": negative dteq member stride");
+    C.ms3 = (long)C.nlev * C.ncol;
+    C.ms2 = C.ncol;
+    C.msq = dteq_mstride;
+    return AER_OK;
+}
+
+int column_launch(aer_sht *h, aer::ColumnArgs &C, int nmem, double *ae, double *an, double *ek, double *dtdt,
+                  double *dlnpsdt, const char *who)
 {
     if (!ae || !an || !ek || !dtdt || !dlnpsdt) return fail(AER_EINVAL, std::string(who) + ": missing array");
     C.ae = ae; C.an = an; C.ek = ek; C.dtdt = dtdt; C.dlnpsdt = dlnpsdt;
     const int nb = (C.ncol + AER_COL_CB - 1) / AER_COL_CB;
     const size_t lds = (size_t)(2 * (C.nlev + 1) + (7 * C.nlev + 1) * AER_COL_CB) * sizeof(double);
-    hipLaunchKernelGGL(aer::k_column_terms, dim3(nb), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
+    hipLaunchKernelGGL(aer::k_column_terms, dim3(nb, nmem), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
@@ -1814,11 +1848,25 @@ int aer_column_terms_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez,
                        const double *phis,
                        double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
 {
+    return aer_column_terms_batch_d(h, vg, 1, held_suarez, nullptr, 0, u, v, vor_g, div_g, t_g, dtdx, dtdy,
+                                    lnps_g, dlnpsdx, dlnpsdy, phis, ae, an, ek, dtdt, dlnpsdt);
+}
+
+int aer_column_terms_batch_d(aer_sht_t h, aer_vgrid_t vg, int nmem, int held_suarez,
+                             const double *dteq, long dteq_mstride,
+                             const double *u, const double *v, const double *vor_g, const double *div_g,
+                             const double *t_g, const double *dtdx, const double *dtdy,
+                             const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                             const double *phis,
+                             double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
+{
     aer::ColumnArgs C;
+    if (dteq && !held_suarez) return fail(AER_EINVAL, "aer_column_terms_batch_d: dteq without the Held-Suarez forcing");
     int rc = column_args(h, vg, held_suarez, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis,
-                         nullptr, C, "aer_column_terms_d");
+                         dteq, C, "aer_column_terms_batch_d");
     if (rc) return rc;
-    return column_launch(h, C, ae, an, ek, dtdt, dlnpsdt, "aer_column_terms_d");
+    if ((rc = column_members(C, nmem, dteq_mstride, "aer_column_terms_batch_d"))) return rc;
+    return column_launch(h, C, nmem, ae, an, ek, dtdt, dlnpsdt, "aer_column_terms_batch_d");
 }
 
 int aer_column_terms_dteq_d(aer_sht_t h, aer_vgrid_t vg, const double *dteq,
@@ -1828,11 +1876,8 @@ int aer_column_terms_dteq_d(aer_sht_t h, aer_vgrid_t vg, const double *dteq,
                             const double *phis,
                             double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt)
 {
-    aer::ColumnArgs C;
-    int rc = column_args(h, vg, 1, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis, dteq, C,
-                         "aer_column_terms_dteq_d");
-    if (rc) return rc;
-    return column_launch(h, C, ae, an, ek, dtdt, dlnpsdt, "aer_column_terms_dteq_d");
+    return aer_column_terms_batch_d(h, vg, 1, 1, dteq, 0, u, v, vor_g, div_g, t_g, dtdx, dtdy,
+                                    lnps_g, dlnpsdx, dlnpsdy, phis, ae, an, ek, dtdt, dlnpsdt);
 }
 
 int aer_column_terms_adj_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez, const double *dteq,
@@ -1846,20 +1891,40 @@ int aer_column_terms_adj_d(aer_sht_t h, aer_vgrid_t vg, int held_suarez, const d
                            double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx, double *g_dlnpsdy,
                            double *g_teq)
 {
+    return aer_column_terms_adj_batch_d(h, vg, 1, held_suarez, dteq, 0, u, v, vor_g, div_g, t_g, dtdx, dtdy,
+                                        lnps_g, dlnpsdx, dlnpsdy, phis, b_ae, b_an, b_ek, b_dtdt, b_dlnpsdt,
+                                        g_u, g_v, g_vor, g_div, g_t, g_dtdx, g_dtdy, g_lnps, g_dlnpsdx, g_dlnpsdy,
+                                        g_teq);
+}
+
+int aer_column_terms_adj_batch_d(aer_sht_t h, aer_vgrid_t vg, int nmem, int held_suarez,
+                                 const double *dteq, long dteq_mstride,
+                                 const double *u, const double *v, const double *vor_g, const double *div_g,
+                                 const double *t_g, const double *dtdx, const double *dtdy,
+                                 const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                                 const double *phis,
+                                 const double *b_ae, const double *b_an, const double *b_ek, const double *b_dtdt,
+                                 const double *b_dlnpsdt,
+                                 double *g_u, double *g_v, double *g_vor, double *g_div, double *g_t,
+                                 double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx,
+                                 double *g_dlnpsdy, double *g_teq)
+{
     aer::ColumnAdjArgs X{};
+    if (dteq && !held_suarez) return fail(AER_EINVAL, "aer_column_terms_adj_batch_d: dteq without the Held-Suarez forcing");
     int rc = column_args(h, vg, held_suarez, u, v, vor_g, div_g, t_g, dtdx, dtdy, lnps_g, dlnpsdx, dlnpsdy, phis,
-                         dteq, X.f, "aer_column_terms_adj_d");
+                         dteq, X.f, "aer_column_terms_adj_batch_d");
     if (rc) return rc;
-    if (vg->nlev > AER_ADJ_MAXLEV) return fail(AER_EINVAL, "aer_column_terms_adj_d: nlev above 64");
+    if ((rc = column_members(X.f, nmem, dteq_mstride, "aer_column_terms_adj_batch_d"))) return rc;
+    if (vg->nlev > AER_ADJ_MAXLEV) return fail(AER_EINVAL, "aer_column_terms_adj_batch_d: nlev above 64");
     if (!b_ae || !b_an || !b_ek || !b_dtdt || !b_dlnpsdt || !g_u || !g_v || !g_vor || !g_div || !g_t || !g_dtdx ||
         !g_dtdy || !g_lnps || !g_dlnpsdx || !g_dlnpsdy)
-        return fail(AER_EINVAL, "aer_column_terms_adj_d: missing array");
+        return fail(AER_EINVAL, "aer_column_terms_adj_batch_d: missing array");
     X.dteq = dteq;
     X.bae = b_ae; X.ban = b_an; X.bek = b_ek; X.bdtdt = b_dtdt; X.bdlnpsdt = b_dlnpsdt;
     X.gu = g_u; X.gv = g_v; X.gvor = g_vor; X.gdiv = g_div; X.gt = g_t; X.gdtdx = g_dtdx; X.gdtdy = g_dtdy;
     X.glnps = g_lnps; X.gdlnpsdx = g_dlnpsdx; X.gdlnpsdy = g_dlnpsdy; X.gteq = g_teq;
     const int nb = (X.f.ncol + 63) / 64;
-    hipLaunchKernelGGL(aer::k_column_terms_adj, dim3(nb), dim3(64), 0, h->stream, X);
+    hipLaunchKernelGGL(aer::k_column_terms_adj, dim3(nb, nmem), dim3(64), 0, h->stream, X);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
@@ -1963,9 +2028,12 @@ int launch_step(aer_semiimp *s, int flags, double eps, double raw_alpha,
                 const double *ov, const double *od, const double *ot, const double *op,
                 double *cv, double *cd, double *ct, double *cp,
                 const double *tv, const double *td, const double *tt, const double *tp,
-                double *nv, double *nd, double *nt, double *np)
+                double *nv, double *nd, double *nt, double *np, int nmem = 1)
 {
+    if (nmem < 1 || nmem > 65535) return fail(AER_EINVAL, "step: member count outside 1..65535");
     aer::StepArgs A{};
+    A.ms3 = 2L * s->h->nlm * s->nlev;
+    A.ms2 = 2L * s->h->nlm;
     A.nlev = s->nlev;
     A.lmax = s->lmax;
     A.nlm = s->h->nlm;
@@ -1992,7 +2060,7 @@ int launch_step(aer_semiimp *s, int flags, double eps, double raw_alpha,
     A.tvor = tv; A.tdiv = td; A.ttemp = tt; A.tlnps = tp;
     A.nvor = nv; A.ndiv = nd; A.ntemp = nt; A.nlnps = np;
     const size_t lds = (size_t)(11 * s->nlev + 5) * AER_SI_CB * sizeof(double);
-    hipLaunchKernelGGL(aer::k_step, dim3(s->nblk), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
+    hipLaunchKernelGGL(aer::k_step, dim3(s->nblk, nmem), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
@@ -2157,6 +2225,17 @@ int aer_leapfrog_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alp
                    const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
                    double *new_vor, double *new_div, double *new_temp, double *new_lnps)
 {
+    return aer_leapfrog_batch_d(s, 1, flags, eps_filter, raw_alpha, old_vor, old_div, old_temp, old_lnps,
+                                now_vor, now_div, now_temp, now_lnps, tnd_vor, tnd_div, tnd_temp, tnd_lnps,
+                                new_vor, new_div, new_temp, new_lnps);
+}
+
+int aer_leapfrog_batch_d(aer_semiimp_t s, int nmem, int flags, double eps_filter, double raw_alpha,
+                         const double *old_vor, const double *old_div, const double *old_temp, const double *old_lnps,
+                         double *now_vor, double *now_div, double *now_temp, double *now_lnps,
+                         const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                         double *new_vor, double *new_div, double *new_temp, double *new_lnps)
+{
     if (!s) return fail(AER_EINVAL, "aer_leapfrog_d: null object");
     if (flags & ~(AER_STEP_SEMI_IMPLICIT | AER_STEP_DIFFUSE | AER_STEP_FILTER))
         return fail(AER_EINVAL, "aer_leapfrog_d: unknown flag");
@@ -2171,7 +2250,7 @@ int aer_leapfrog_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alp
     if ((flags & AER_STEP_FILTER) && eps_filter > 0.0) f |= aer::ST_RAW;
     return launch_step(s, f, eps_filter, raw_alpha, old_vor, old_div, old_temp, old_lnps,
                        now_vor, now_div, now_temp, now_lnps, tnd_vor, tnd_div, tnd_temp, tnd_lnps,
-                       new_vor, new_div, new_temp, new_lnps);
+                       new_vor, new_div, new_temp, new_lnps, nmem);
 }
 
 int aer_leapfrog_adj_d(aer_semiimp_t s, int flags, double eps_filter, double raw_alpha,
@@ -2181,7 +2260,20 @@ int aer_leapfrog_adj_d(aer_semiimp_t s, int flags, double eps_filter, double raw
                        double *gc_vor, double *gc_div, double *gc_temp, double *gc_lnps,
                        double *gt_vor, double *gt_div, double *gt_temp, double *gt_lnps)
 {
+    return aer_leapfrog_adj_batch_d(s, 1, flags, eps_filter, raw_alpha, bn_vor, bn_div, bn_temp, bn_lnps,
+                                    bc_vor, bc_div, bc_temp, bc_lnps, go_vor, go_div, go_temp, go_lnps,
+                                    gc_vor, gc_div, gc_temp, gc_lnps, gt_vor, gt_div, gt_temp, gt_lnps);
+}
+
+int aer_leapfrog_adj_batch_d(aer_semiimp_t s, int nmem, int flags, double eps_filter, double raw_alpha,
+                             const double *bn_vor, const double *bn_div, const double *bn_temp, const double *bn_lnps,
+                             const double *bc_vor, const double *bc_div, const double *bc_temp, const double *bc_lnps,
+                             double *go_vor, double *go_div, double *go_temp, double *go_lnps,
+                             double *gc_vor, double *gc_div, double *gc_temp, double *gc_lnps,
+                             double *gt_vor, double *gt_div, double *gt_temp, double *gt_lnps)
+{
     if (!s) return fail(AER_EINVAL, "aer_leapfrog_adj_d: null object");
+    if (nmem < 1 || nmem > 65535) return fail(AER_EINVAL, "aer_leapfrog_adj_d: member count outside 1..65535");
     if (flags & ~(AER_STEP_SEMI_IMPLICIT | AER_STEP_DIFFUSE | AER_STEP_FILTER))
         return fail(AER_EINVAL, "aer_leapfrog_adj_d: unknown flag");
     if ((flags & AER_STEP_DIFFUSE) && !s->diff)
@@ -2220,31 +2312,58 @@ int aer_leapfrog_adj_d(aer_semiimp_t s, int flags, double eps_filter, double raw
     A.govor = go_vor; A.godiv = go_div; A.gotemp = go_temp; A.golnps = go_lnps;
     A.gcvor = gc_vor; A.gcdiv = gc_div; A.gctemp = gc_temp; A.gclnps = gc_lnps;
     A.gtvor = gt_vor; A.gtdiv = gt_div; A.gttemp = gt_temp; A.gtlnps = gt_lnps;
+    A.ms3 = 2L * s->h->nlm * s->nlev;
+    A.ms2 = 2L * s->h->nlm;
     const size_t lds = (size_t)(5 * s->nlev + 3) * AER_SI_CB * sizeof(double);
-    hipLaunchKernelGGL(aer::k_step_adj, dim3(s->nblk), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
+    hipLaunchKernelGGL(aer::k_step_adj, dim3(s->nblk, nmem), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
 
-int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out)
+}  // extern "C"
+
+namespace {
+
+int mass_impl(aer_sht *h, int nmem, double *lnps, int fix, double target, const double *targets,
+              double *mass_out, double *lnr_out, const char *who)
 {
-    if (!h) return fail(AER_EINVAL, "aer_mass_d: null handle");
-    if (!h->grid_set) return fail(AER_ESTATE, "aer_mass_d: handle without a grid");
-    if (!lnps) return fail(AER_EINVAL, "aer_mass_d: missing array");
-    if (fix && mass_target <= 0.0) return fail(AER_EINVAL, "aer_mass_d: mass target must be > 0");
-    JobSpec sp{aer::SYN_SCALAR, 1};
+    if (!h) return fail(AER_EINVAL, std::string(who) + ": null handle");
+    if (!h->grid_set) return fail(AER_ESTATE, std::string(who) + ": handle without a grid");
+    if (!lnps) return fail(AER_EINVAL, std::string(who) + ": missing array");
+    if (nmem < 1 || nmem > 65535) return fail(AER_EINVAL, std::string(who) + ": member count outside 1..65535");
+    const long ng = (long)h->nlon * h->nlat;
+    // grows once per member count; a caller that captures graphs makes its first call outside capture
+    int rc = h->massg.ensure((size_t)nmem * ng * sizeof(double));
+    if (rc) return rc;
+    JobSpec sp{aer::SYN_SCALAR, nmem};
     sp.sin0 = lnps;
     sp.sstride = 2L * h->nlm;
     sp.gout0 = (double *)h->massg.p;
-    sp.gstride = (long)h->nlon * h->nlat;
+    sp.gstride = ng;
     sp.prec = 0;   // global mass sums stay FP64
-    int rc = run_jobs(h, &sp, 1, true);
-    if (rc) return rc;
-    hipLaunchKernelGGL(aer::k_mass, dim3(1), dim3(256), (size_t)5 * h->nlat * sizeof(double), h->stream,
+    if ((rc = run_jobs(h, &sp, 1, true))) return rc;
+    hipLaunchKernelGGL(aer::k_mass, dim3(nmem), dim3(256), (size_t)5 * h->nlat * sizeof(double), h->stream,
                        (const double *)h->massg.p, h->nlon, h->nlat, (const double *)h->wfull.p, 9.80665,
-                       fix ? 1 : 0, mass_target, h->c00, lnps, mass_out, lnr_out);
+                       fix ? 1 : 0, target, targets, h->c00, lnps, mass_out, lnr_out, ng, 2L * h->nlm);
     HIP_TRY(hipGetLastError());
     return AER_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aer_mass_d(aer_sht_t h, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out)
+{
+    if (fix && mass_target <= 0.0) return fail(AER_EINVAL, "aer_mass_d: mass target must be > 0");
+    return mass_impl(h, 1, lnps, fix, mass_target, nullptr, mass_out, lnr_out, "aer_mass_d");
+}
+
+int aer_mass_batch_d(aer_sht_t h, int nmem, double *lnps, int fix, const double *mass_targets,
+                     double *mass_out, double *lnr_out)
+{
+    if (fix && !mass_targets) return fail(AER_EINVAL, "aer_mass_batch_d: fix needs the device array of targets");
+    return mass_impl(h, nmem, lnps, fix, 0.0, mass_targets, mass_out, lnr_out, "aer_mass_batch_d");
 }
 
 }  // extern "C"

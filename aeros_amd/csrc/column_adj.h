@@ -32,8 +32,23 @@ struct ColumnAdjArgs {
     double *gteq;                                    // [nlev][ncol]: d/d(dteq) per column, or null
 };
 
-__global__ __launch_bounds__(64) void k_column_terms_adj(const ColumnAdjArgs X)
+__device__ __forceinline__ ColumnAdjArgs member_args(ColumnAdjArgs X)
 {
+    const long m = blockIdx.y, m3 = m * X.f.ms3, m2 = m * X.f.ms2;
+    col_shift(X.dteq, m * X.f.msq);
+    X.f = member_args(X.f);
+    col_shift(X.bae, m3); col_shift(X.ban, m3); col_shift(X.bek, m3); col_shift(X.bdtdt, m3);
+    col_shift(X.bdlnpsdt, m2);
+    col_shift(X.gu, m3); col_shift(X.gv, m3); col_shift(X.gvor, m3); col_shift(X.gdiv, m3);
+    col_shift(X.gt, m3); col_shift(X.gdtdx, m3); col_shift(X.gdtdy, m3);
+    col_shift(X.glnps, m2); col_shift(X.gdlnpsdx, m2); col_shift(X.gdlnpsdy, m2);
+    col_shift(X.gteq, m3);
+    return X;
+}
+
+__global__ __launch_bounds__(64) void k_column_terms_adj(const ColumnAdjArgs X0)
+{
+    const ColumnAdjArgs X = member_args(X0);
     const ColumnArgs &C = X.f;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= C.ncol) return;

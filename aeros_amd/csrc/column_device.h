@@ -39,7 +39,27 @@ struct ColumnArgs {
     double *ae, *an, *ek, *dtdt;    // [nlev][ncol]
     double *dlnpsdt;                // [ncol]
     const double *dteq;             // optional additive Held-Suarez T_eq [nlev][nlat], or null
+    long ms3, ms2, msq;             // member strides (doubles): 3-D fields, 2-D fields, dteq (0: shared)
 };
+
+// Ensemble members of one size share the grid and the vertical coordinate:
+// blockIdx.y selects the member and its fields sit at fixed strides (phis is shared).
+template <typename P>
+__device__ __forceinline__ void col_shift(P *&p, long off)
+{
+    if (p) p += off;
+}
+__device__ __forceinline__ ColumnArgs member_args(ColumnArgs C)
+{
+    const long m = blockIdx.y, m3 = m * C.ms3, m2 = m * C.ms2;
+    col_shift(C.u, m3); col_shift(C.v, m3); col_shift(C.vor, m3); col_shift(C.div, m3);
+    col_shift(C.t, m3); col_shift(C.dtdx, m3); col_shift(C.dtdy, m3);
+    col_shift(C.lnps, m2); col_shift(C.dlnpsdx, m2); col_shift(C.dlnpsdy, m2);
+    col_shift(C.ae, m3); col_shift(C.an, m3); col_shift(C.ek, m3); col_shift(C.dtdt, m3);
+    col_shift(C.dlnpsdt, m2);
+    col_shift(C.dteq, m * C.msq);
+    return C;
+}
 
 // Per-level column geometry: dp, alpha, dlnp (with the zero-top convention of
 // src/dynamics/aeros_tendency.f90:514-519) and c_k.
@@ -65,8 +85,9 @@ __device__ __forceinline__ LevelGeo level_geo(const ColumnArgs &C, const double 
     return g;
 }
 
-__global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C)
+__global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
 {
+    const ColumnArgs C = member_args(C0);
     extern __shared__ double lds[];
     const int nlev = C.nlev;
     const int cl = threadIdx.x % AER_COL_CB, kl = threadIdx.x / AER_COL_CB;

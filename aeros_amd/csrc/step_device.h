@@ -35,10 +35,29 @@ struct StepArgs {
     double *cvor, *cdiv, *ctemp, *clnps;         // X^n (the filter updates it in place)
     const double *tvor, *tdiv, *ttemp, *tlnps;   // R(X^n)
     double *nvor, *ndiv, *ntemp, *nlnps;         // X^(n+1); with ST_LINEAR: L(X^n) of div, temp, lnps
+    long ms3, ms2;                               // member strides (doubles) of 3-D and ln p_s arrays
 };
 
-__global__ __launch_bounds__(256) void k_step(const StepArgs S)
+// Ensemble members of one size are independent copies of the same operators:
+// blockIdx.y selects the member and every state array moves by its stride.
+template <typename P>
+__device__ __forceinline__ void member_shift(P *&p, long off)
 {
+    if (p) p += off;
+}
+__device__ __forceinline__ StepArgs member_args(StepArgs S)
+{
+    const long m3 = (long)blockIdx.y * S.ms3, m2 = (long)blockIdx.y * S.ms2;
+    member_shift(S.ovor, m3); member_shift(S.odiv, m3); member_shift(S.otemp, m3); member_shift(S.olnps, m2);
+    member_shift(S.cvor, m3); member_shift(S.cdiv, m3); member_shift(S.ctemp, m3); member_shift(S.clnps, m2);
+    member_shift(S.tvor, m3); member_shift(S.tdiv, m3); member_shift(S.ttemp, m3); member_shift(S.tlnps, m2);
+    member_shift(S.nvor, m3); member_shift(S.ndiv, m3); member_shift(S.ntemp, m3); member_shift(S.nlnps, m2);
+    return S;
+}
+
+__global__ __launch_bounds__(256) void k_step(const StepArgs S0)
+{
+    const StepArgs S = member_args(S0);
     extern __shared__ double lds[];
     const int n = S.nlev;
     const int c = threadIdx.x % AER_SI_CB, kl = threadIdx.x / AER_SI_CB;
@@ -231,11 +250,17 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S)
 
 // Global dry-air mass of ln p_s on the grid (src/dynamics/aeros_timestep.f90:1250-1276,
 // the quadrature of src/aeros_spectral.f90:632-666: row sums first, rows summed in
-// order, FP64), then, if fix, ln p_s(0,0) += ln(target/mass) c00. One block.
+// order, FP64), then, if fix, ln p_s(0,0) += ln(target/mass) c00. One block per
+// ensemble member (blockIdx.x): member b reads grid b (stride gs) and fixes spectral
+// field b (stride ss) toward targets[b], or toward `target` when targets is null.
 __global__ __launch_bounds__(256) void k_mass(const double *lnps_g, int nlon, int nlat, const double *w,
-                                              double grav, int fix, double target, double c00,
-                                              double *lnps, double *mass_out, double *lnr_out)
+                                              double grav, int fix, double target, const double *targets,
+                                              double c00, double *lnps, double *mass_out, double *lnr_out,
+                                              long gs, long ss)
 {
+    const int b = blockIdx.x;
+    lnps_g += (long)b * gs;
+    lnps += (long)b * ss;
     extern __shared__ double rows[];   // nlat row totals, then 4 partials per row
     double *part = rows + nlat;
     const double dlon = 2.0 * 3.14159265358979323846264338327950288 / (double)nlon;
@@ -258,11 +283,11 @@ __global__ __launch_bounds__(256) void k_mass(const double *lnps_g, int nlon, in
     double total = 0.0;
     for (int j = 0; j < nlat; ++j) total = total + rows[j];
     const double mass = total * 6.371e6 * 6.371e6 / grav;
-    if (mass_out) mass_out[0] = mass;
+    if (mass_out) mass_out[b] = mass;
     if (fix) {
-        const double lnr = log(target / mass);
+        const double lnr = log((targets ? targets[b] : target) / mass);
         lnps[0] = lnps[0] + lnr * c00;
-        if (lnr_out) lnr_out[0] = lnr;
+        if (lnr_out) lnr_out[b] = lnr;
     }
 }
 
@@ -284,10 +309,23 @@ struct StepAdjArgs {
     double *govor, *godiv, *gotemp, *golnps;         // adjoint of X^(n-1)
     double *gcvor, *gcdiv, *gctemp, *gclnps;         // adjoint of X^n (the input, before the filter)
     double *gtvor, *gtdiv, *gttemp, *gtlnps;         // adjoint of the tendency R(X^n)
+    long ms3, ms2;                                   // member strides, as in StepArgs
 };
 
-__global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S)
+__device__ __forceinline__ StepAdjArgs member_args(StepAdjArgs S)
 {
+    const long m3 = (long)blockIdx.y * S.ms3, m2 = (long)blockIdx.y * S.ms2;
+    member_shift(S.bnvor, m3); member_shift(S.bndiv, m3); member_shift(S.bntemp, m3); member_shift(S.bnlnps, m2);
+    member_shift(S.bcvor, m3); member_shift(S.bcdiv, m3); member_shift(S.bctemp, m3); member_shift(S.bclnps, m2);
+    member_shift(S.govor, m3); member_shift(S.godiv, m3); member_shift(S.gotemp, m3); member_shift(S.golnps, m2);
+    member_shift(S.gcvor, m3); member_shift(S.gcdiv, m3); member_shift(S.gctemp, m3); member_shift(S.gclnps, m2);
+    member_shift(S.gtvor, m3); member_shift(S.gtdiv, m3); member_shift(S.gttemp, m3); member_shift(S.gtlnps, m2);
+    return S;
+}
+
+__global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
+{
+    const StepAdjArgs S = member_args(S0);
     extern __shared__ double lds[];
     const int n = S.nlev;
     const int c = threadIdx.x % AER_SI_CB, kl = threadIdx.x / AER_SI_CB;

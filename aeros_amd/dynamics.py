@@ -49,11 +49,26 @@ class Stepper:
         self.now = self.old = None
 
     def set_state(self, vor, div, temp, lnps):
-        """X^n from spectral (nlev, nlm) / (nlm,) arrays (numpy or CUDA tensors)."""
+        """X^n from spectral (nlev, nlm) / (nlm,) arrays (numpy or CUDA tensors). An
+        ensemble of members of this size passes (nmem, nlev, nlm) / (nmem, nlm): every
+        launch of a step then covers all members."""
         dev = lambda a: torch.as_tensor(a, dtype=torch.complex128).cuda().contiguous()
-        self.now = {"vor": dev(vor), "div": dev(div), "temp": dev(temp), "lnps": dev(lnps)}
+        now = {"vor": dev(vor), "div": dev(div), "temp": dev(temp), "lnps": dev(lnps)}
+        self.si._members(now)
+        self.now = now
         self.old = None
         self.nstep = 0
+        self.mass_target = None
+        self.lnr_cum = None
+
+    @property
+    def members(self):
+        """Ensemble size, or 0 for a single model."""
+        return self.now["vor"].shape[0] if self.now is not None and self.now["vor"].dim() == 3 else 0
+
+    def _mass_target(self, lnps):
+        m = mass(self.sht, lnps)[0]
+        return m.item() if m.dim() == 0 else m.clone()
 
     def step(self):
         s, now = self.sht, self.now
@@ -61,7 +76,7 @@ class Stepper:
             h = self.dt
             self.old = {k: v.clone() for k, v in now.items()}
             if self.mass_fixer and self.mass_target is None:
-                self.mass_target = mass(s, now["lnps"])[0].item()
+                self.mass_target = self._mass_target(now["lnps"])
         else:
             h = 2.0 * self.dt
         if h != self.h:
@@ -95,7 +110,7 @@ class Stepper:
                 "now": {k: v.clone() for k, v in self.now.items()},
                 "mass_target": self.mass_target,
                 "lnr_cum": None if self.lnr_cum is None else self.lnr_cum.clone(),
-                "shape": (self.nlev, self.sht.nlm, self.sht.nlat, self.sht.nlon)}
+                "shape": (self.nlev, self.sht.nlm, self.sht.nlat, self.sht.nlon), "members": self.members}
 
     def restore(self, ck):
         """Resume from checkpoint(); the shapes must match this stepper's."""
@@ -103,6 +118,8 @@ class Stepper:
             raise ValueError("checkpoint version mismatch")
         if tuple(ck["shape"]) != (self.nlev, self.sht.nlm, self.sht.nlat, self.sht.nlon):
             raise ValueError(f"checkpoint of shape {tuple(ck['shape'])} does not fit this stepper")
+        if self.now is not None and int(ck.get("members", 0)) != self.members:
+            raise ValueError("checkpoint of another ensemble size")
         self.now = {k: v.clone() for k, v in ck["now"].items()}
         self.old = None if ck["old"] is None else {k: v.clone() for k, v in ck["old"].items()}
         self.nstep = int(ck["nstep"])
@@ -114,10 +131,10 @@ def save_checkpoint(ck, path):
     """A checkpoint to a .npz file: spectral levels as complex128 arrays, scalars and the
     shape (nlev, nlm, nlat, nlon) as metadata. Loaded without pickle."""
     import numpy as np
+    host = lambda x: np.array(np.nan) if x is None else (x.cpu().numpy() if torch.is_tensor(x) else np.array(x))
     arrs = {"version": np.array(ck["version"]), "nstep": np.array(ck["nstep"]),
-            "shape": np.array(ck["shape"], dtype=np.int64),
-            "mass_target": np.array(np.nan if ck["mass_target"] is None else ck["mass_target"]),
-            "lnr_cum": np.array(np.nan if ck["lnr_cum"] is None else float(ck["lnr_cum"]))}
+            "shape": np.array(ck["shape"], dtype=np.int64), "members": np.array(int(ck.get("members", 0))),
+            "mass_target": host(ck["mass_target"]), "lnr_cum": host(ck["lnr_cum"])}
     for lev in ("old", "now"):
         if ck[lev] is not None:
             for k, v in ck[lev].items():
@@ -132,8 +149,14 @@ def load_checkpoint(path, device="cuda"):
     for lev in ("old", "now"):
         keys = [k for k in HS_KEYS if f"{lev}_{k}" in z.files]
         levs[lev] = {k: torch.from_numpy(z[f"{lev}_{k}"]).to(device) for k in keys} if keys else None
-    mt, lc = float(z["mass_target"]), float(z["lnr_cum"])
+    members = int(z["members"]) if "members" in z.files else 0
+    mt, lc = z["mass_target"], z["lnr_cum"]
+    if members:   # per-member device arrays
+        mass_target = None if mt.ndim == 0 else torch.from_numpy(mt).to(device)
+        lnr_cum = None if lc.ndim == 0 else torch.from_numpy(lc).to(device)
+    else:
+        mass_target = None if np.isnan(float(mt)) else float(mt)
+        lnr_cum = None if np.isnan(float(lc)) else torch.tensor(float(lc), dtype=torch.float64, device=device)
     return {"version": int(z["version"]), "nstep": int(z["nstep"]), "old": levs["old"], "now": levs["now"],
-            "mass_target": None if np.isnan(mt) else mt,
-            "lnr_cum": None if np.isnan(lc) else torch.tensor(lc, dtype=torch.float64, device=device),
+            "mass_target": mass_target, "lnr_cum": lnr_cum, "members": members,
             "shape": tuple(int(x) for x in z["shape"])}

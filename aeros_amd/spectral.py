@@ -321,33 +321,45 @@ class AerosSht:
     # ------------------------------------------------------------ tendency halves (device)
     def tendency_grid_transforms(self, vor, div, temp, lnps, out=None):
         """Spectral -> grid half of aeros_tendency_grid (src/dynamics/aeros_tendency.f90:403-419).
-        vor/div/temp: (nlev, nlm); lnps: (nlm,). Returns a dict of grid tensors."""
-        nlev = vor.shape[0]
+        vor/div/temp: (nlev, nlm); lnps: (nlm,). Returns a dict of grid tensors. An ensemble
+        of members of this size passes (nmem, nlev, nlm) and (nmem, nlm): one fused call."""
+        lead = tuple(vor.shape[:-2])
+        if len(lead) > 1 or tuple(lnps.shape) != lead + (self.nlm,):
+            raise ValueError("vor/div/temp must be ([nmem,] nlev, nlm) and lnps ([nmem,] nlm)")
+        nmem, nlev = (lead[0] if lead else 1), vor.shape[-2]
         for a in (vor, div, temp):
             self._dev_spec(a)
+            if a.shape != vor.shape:
+                raise ValueError("vor, div and temp must have the same shape")
         self._dev_spec(lnps)
         if out is None:
-            out = {k: self._new_grid(vor, (nlev,)) for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy")}
-            out.update({k: self._new_grid(vor, ()) for k in ("lnps_g", "dlnpsdx", "dlnpsdy")})
+            out = {k: self._new_grid(vor, lead + (nlev,)) for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy")}
+            out.update({k: self._new_grid(vor, lead) for k in ("lnps_g", "dlnpsdx", "dlnpsdy")})
         self._bind_stream()
-        check(lib().aer_tendency_grid_transforms_d(
-            self._h, nlev, vor.data_ptr(), div.data_ptr(), temp.data_ptr(), lnps.data_ptr(),
+        check(lib().aer_tendency_grid_transforms_batch_d(
+            self._h, nmem, nlev, vor.data_ptr(), div.data_ptr(), temp.data_ptr(), lnps.data_ptr(),
             *[out[k].data_ptr() for k in ("u", "v", "vor_g", "div_g", "t_g", "dtdx", "dtdy",
                                           "lnps_g", "dlnpsdx", "dlnpsdy")]))
         return out
 
     def tendency_spectral(self, ae, an, ek, dtdt, dlnpsdt, out=None):
-        """aeros_tendency_spectral (src/dynamics/aeros_tendency.f90:428-473); inputs preserved."""
-        nlev = ae.shape[0]
+        """aeros_tendency_spectral (src/dynamics/aeros_tendency.f90:428-473); inputs preserved.
+        ae/an/ek/dtdt: ([nmem,] nlev, nlat, nlon); dlnpsdt ([nmem,] nlat, nlon)."""
+        lead = tuple(ae.shape[:-3])
+        if len(lead) > 1 or tuple(dlnpsdt.shape) != lead + (self.nlat, self.nlon):
+            raise ValueError("ae/an/ek/dtdt must be ([nmem,] nlev, nlat, nlon) and dlnpsdt ([nmem,] nlat, nlon)")
+        nmem, nlev = (lead[0] if lead else 1), ae.shape[-3]
         for a in (ae, an, ek, dtdt):
             self._dev_grid(a)
+            if a.shape != ae.shape:
+                raise ValueError("ae, an, ek and dtdt must have the same shape")
         self._dev_grid(dlnpsdt)
         if out is None:
-            out = {k: self._new_spec(ae, (nlev,)) for k in ("vor", "div", "temp")}
-            out["lnps"] = self._new_spec(ae, ())
+            out = {k: self._new_spec(ae, lead + (nlev,)) for k in ("vor", "div", "temp")}
+            out["lnps"] = self._new_spec(ae, lead)
         self._bind_stream()
-        check(lib().aer_tendency_spectral_d(
-            self._h, nlev, ae.data_ptr(), an.data_ptr(), ek.data_ptr(), dtdt.data_ptr(), dlnpsdt.data_ptr(),
+        check(lib().aer_tendency_spectral_batch_d(
+            self._h, nmem, nlev, ae.data_ptr(), an.data_ptr(), ek.data_ptr(), dtdt.data_ptr(), dlnpsdt.data_ptr(),
             out["vor"].data_ptr(), out["div"].data_ptr(), out["temp"].data_ptr(), out["lnps"].data_ptr()))
         return out
 

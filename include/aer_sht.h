@@ -173,6 +173,19 @@ int aer_tendency_spectral_d(aer_sht_t cfg, int nlev,
         const double *ae, const double *an, const double *ek, const double *dtdt,
         const double *dlnpsdt,
         double *tvor, double *tdiv, double *ttemp, double *tlnps);
+/* Ensemble forms of the two halves: nmem members of this size, member b's 3-D arrays
+ * at b*nlev fields and its single-level arrays at b fields from the first pointer
+ * ([nmem][nlev][...] and [nmem][...] contiguous). The reference steps one model per
+ * call (src/dynamics/aeros_tendency.f90:362-473); members batch into the same fused
+ * launches as levels do. */
+int aer_tendency_grid_transforms_batch_d(aer_sht_t cfg, int nmem, int nlev,
+        const double *vor, const double *div, const double *temp, const double *lnps,
+        double *u, double *v, double *vor_g, double *div_g, double *t_g,
+        double *dtdx, double *dtdy, double *lnps_g, double *dlnpsdx, double *dlnpsdy);
+int aer_tendency_spectral_batch_d(aer_sht_t cfg, int nmem, int nlev,
+        const double *ae, const double *an, const double *ek, const double *dtdt,
+        const double *dlnpsdt,
+        double *tvor, double *tdiv, double *ttemp, double *tlnps);
 
 /* ---- fused launches: a list of batched transforms of one direction run as ONE
  * launch per stage (the Legendre GEMM kernel and the FFT kernel). Every entry
@@ -306,6 +319,30 @@ int aer_column_terms_adj_d(aer_sht_t cfg, aer_vgrid_t vg, int held_suarez, const
                            double *g_u, double *g_v, double *g_vor, double *g_div, double *g_t,
                            double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx, double *g_dlnpsdy,
                            double *g_teq);
+/* Ensemble forms of the column terms and their adjoint: nmem members of this size,
+ * grid arrays [nmem][nlev][nlat][nlon] and [nmem][nlat][nlon], one launch (the member
+ * is the grid's second dimension, at most 65535). dteq (may be NULL) is member b's at
+ * b*dteq_mstride doubles (0: one field shared by all members); phis is shared. With
+ * nmem = 1 and dteq_mstride = 0 they are aer_column_terms_d / _dteq_d / _adj_d.
+ * No reference counterpart (the reference steps one model per call). */
+int aer_column_terms_batch_d(aer_sht_t cfg, aer_vgrid_t vg, int nmem, int held_suarez,
+                             const double *dteq, long dteq_mstride,
+                             const double *u, const double *v, const double *vor_g, const double *div_g,
+                             const double *t_g, const double *dtdx, const double *dtdy,
+                             const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                             const double *phis,
+                             double *ae, double *an, double *ek, double *dtdt, double *dlnpsdt);
+int aer_column_terms_adj_batch_d(aer_sht_t cfg, aer_vgrid_t vg, int nmem, int held_suarez,
+                                 const double *dteq, long dteq_mstride,
+                                 const double *u, const double *v, const double *vor_g, const double *div_g,
+                                 const double *t_g, const double *dtdx, const double *dtdy,
+                                 const double *lnps_g, const double *dlnpsdx, const double *dlnpsdy,
+                                 const double *phis,
+                                 const double *b_ae, const double *b_an, const double *b_ek, const double *b_dtdt,
+                                 const double *b_dlnpsdt,
+                                 double *g_u, double *g_v, double *g_vor, double *g_div, double *g_t,
+                                 double *g_dtdx, double *g_dtdy, double *g_lnps, double *g_dlnpsdx,
+                                 double *g_dlnpsdy, double *g_teq);
 
 /* ---- spectral step completion: the semi-implicit gravity-wave solve
  * (src/dynamics/aeros_semiimp.f90) and the leapfrog pieces of
@@ -374,6 +411,26 @@ int aer_leapfrog_adj_d(aer_semiimp_t si, int flags, double eps_filter, double ra
  * ln p_s(0,0) += ln(mass_target/mass) c00, ln r written to *lnr_out (device, may be
  * NULL). c00 is read back from an analysis of the constant 1 at aer_sht_set_grid. */
 int aer_mass_d(aer_sht_t cfg, double *lnps, int fix, double mass_target, double *mass_out, double *lnr_out);
+
+/* Ensemble forms of the leapfrog, its adjoint and the mass fixer: nmem members of this
+ * size, spectral arrays [nmem][nlev][nlm] and [nmem][nlm] (complex), one launch each.
+ * aer_mass_batch_d reads member b's target from the DEVICE array mass_targets[b] and
+ * writes mass_out[b], lnr_out[b] (device, may be NULL); its first call at a given
+ * member count sizes the handle's grid buffer, so make that call outside graph capture.
+ * No reference counterpart (the reference steps one model per call). */
+int aer_leapfrog_batch_d(aer_semiimp_t si, int nmem, int flags, double eps_filter, double raw_alpha,
+                         const double *old_vor, const double *old_div, const double *old_temp, const double *old_lnps,
+                         double *now_vor, double *now_div, double *now_temp, double *now_lnps,
+                         const double *tnd_vor, const double *tnd_div, const double *tnd_temp, const double *tnd_lnps,
+                         double *new_vor, double *new_div, double *new_temp, double *new_lnps);
+int aer_leapfrog_adj_batch_d(aer_semiimp_t si, int nmem, int flags, double eps_filter, double raw_alpha,
+                             const double *bn_vor, const double *bn_div, const double *bn_temp, const double *bn_lnps,
+                             const double *bc_vor, const double *bc_div, const double *bc_temp, const double *bc_lnps,
+                             double *go_vor, double *go_div, double *go_temp, double *go_lnps,
+                             double *gc_vor, double *gc_div, double *gc_temp, double *gc_lnps,
+                             double *gt_vor, double *gt_div, double *gt_temp, double *gt_lnps);
+int aer_mass_batch_d(aer_sht_t cfg, int nmem, double *lnps, int fix, const double *mass_targets,
+                     double *mass_out, double *lnr_out);
 
 /* ---- zonal-mean statistics: replaces aeros_diag_init / aeros_diag_accumulate /
  * aeros_diag_finalize's sums (src/aeros_diagnostics.f90:114-276) with FP64 running sums on

@@ -213,6 +213,11 @@ class TransformSet:
         return out
 
 
+def torch_real(z):
+    import torch
+    return torch.view_as_real(z)
+
+
 def time_steps(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -349,6 +354,44 @@ def ensemble_config2(torch, precision, steps=50, warmup=5, path="auto"):
             "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
+def ensemble_config2_step(torch, precision="fp32", steps=50, warmup=5, days=1.0):
+    """Whole Held-Suarez steps of the 64-member uneven ensemble (24 x T21L16, 24 x T31L20,
+    16 x T42L20) from the seeded rest state: the transforms of all members in two autotuned
+    plans, column terms, FP64 leapfrog and mass fixer batched per size, every step after
+    the second a HIP-graph replay. Then `days` of integration as a stability check."""
+    from aeros_amd.ensemble import EnsembleStepper, rest_states
+    mix = [(21, 16, 24), (31, 20, 24), (42, 20, 16)]
+    ens = EnsembleStepper(mix, dt=1200.0, precision=precision)
+    tuned = ens.autotune()
+    ens.set_state(rest_states(mix))
+    ens.step()
+    ens.step()
+    ens.capture()
+    ms = time_steps(torch, ens.step, steps, warmup)
+    nday = int(round(days * 86400.0 / ens.dt))
+    while ens.nstep < nday:
+        ens.step()
+    torch.cuda.synchronize()
+    vmax = max(float(ens.now(i)["vor"].abs().max()) for i in range(len(mix)))
+    ntr = sum(n * (11 * L + 4) for _, L, n in mix)
+    nbytes = 0
+    for T_, L_, n in mix:
+        from aeros_amd.spectral import grid_size
+        from aeros_amd.seeding import lm_tables
+        nlon, nlat = grid_size(T_)
+        nbytes += n * sref_bytes(nlon, nlat, lm_tables(T_)[0].size, L_)
+    names = lambda c: [TUNE_NAMES[i] for i in c]
+    return {"workload": "64 Held-Suarez members (24 x T21L16 + 24 x T31L20 + 16 x T42L20), whole dynamics steps: "
+                        "2 fused transform plans + per-size column terms, FP64 leapfrog and mass fixer, "
+                        "HIP-graph replay per step",
+            "precision": precision + (" (assembly job fp64)" if precision == "fp32" else ""),
+            "ms_per_ensemble_step": round(ms, 4), "member_steps_per_s": round(64 * 1000.0 / ms, 0),
+            "sref_transforms_per_s": round(ntr * 1000.0 / ms, 0),
+            "transform_alg_hbm_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "autotune_kept": {"synthesis": names(tuned["synthesis"][1])[0], "analysis": names(tuned["analysis"][1])[1]},
+            "stability_days": days, "max_abs_vor_after": vmax}
+
+
 def stress_T106(torch, members=32, nlev=30, steps=5, warmup=1):
     """The stress configuration's transform work: the S_ref set (11L+4 transforms) of 32
     T106L30 members (Gaussian 160 x 320) in one compiled plan, split-FP32 MFMA Legendre, and
@@ -430,8 +473,78 @@ def stress_T106(torch, members=32, nlev=30, steps=5, warmup=1):
     return res
 
 
+def stress_day_T106(torch, members=32, nlev=30, dt=600.0, hours=24.0, every=16, precisions=("split", "fp64")):
+    """The stress configuration end to end: 32 T106L30 Held-Suarez members batched in one
+    Stepper (every launch covers all members), one model day forward at dt = 600 s and the
+    checkpointed gradient of the summed objective (checkpoints every 16 steps), in the
+    split-FP32 form and in FP64. Reports ms per step, S_ref transforms/s and their HBM rate,
+    peak device memory, and the split form's per-field error of the final state and of the
+    gradient against the FP64 run."""
+    from aeros_amd.adjoint import HeldSuarezAdjoint
+    from aeros_amd.column import vgrid_init
+    from aeros_amd.dynamics import Stepper
+    from aeros_amd.seeding import red_spectra
+    T = 106
+    nsteps = int(round(hours * 3600.0 / dt))
+    A, B = vgrid_init(nlev, stretch_a=1.0, stretch_r=2.0, sigma_t=0.0, p_top=1000.0)
+    out, finals, grads = {}, {}, {}
+    for prec in precisions:
+        st = Stepper(T, A, B, dt, tau_diff=6 * 3600.0, ndiff=6, eps_filter=0.06, raw_alpha=0.53,
+                     held_suarez=True, mass_fixer=True, precision=prec)
+        s = st.sht
+        l_of = torch.as_tensor(s.l_of_lm, device="cuda")
+        # rest, isothermal 300 K, p_s = 1000 hPa, seeded vorticity noise of grid rms 1e-6 s^-1 on l <= 20
+        vor = torch.as_tensor(np.stack([red_spectra(T, nlev, 4, member=m, field0=0, zero_l0=True)
+                                        for m in range(members)]), dtype=torch.complex128).cuda()
+        vor = torch.where(l_of <= 20, vor, torch.zeros_like(vor)).contiguous()
+        s.set_precision("fp64")
+        vor *= 1e-6 / float(s.synthesis(vor[0]).pow(2).mean().sqrt())
+        s.set_precision(prec)
+        temp = torch.zeros_like(vor)
+        temp[..., 0] = 300.0 * np.sqrt(4 * np.pi)
+        lnps = torch.zeros((members, s.nlm), dtype=torch.complex128, device="cuda")
+        lnps[:, 0] = np.log(1.0e5) * np.sqrt(4 * np.pi)
+        x0 = [vor, torch.zeros_like(vor), temp, lnps]
+        st.dteq = torch.zeros((members, nlev, s.nlat), dtype=torch.float64, device="cuda")
+        adj = HeldSuarezAdjoint(st)
+        adj.run(x0, 2)                      # warm-up: tables, plans, buffers
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        t0 = time.perf_counter()
+        J = adj.run(x0, nsteps)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        finals[prec] = {k: v.clone() for k, v in st.now.items()}
+        t0 = time.perf_counter()
+        Jg, lam, gq = adj.gradient(x0, nsteps, every=every)
+        torch.cuda.synchronize()
+        t_grad = time.perf_counter() - t0
+        free, total = torch.cuda.mem_get_info()
+        grads[prec] = dict(lam, teq=gq)
+        ntr = members * (11 * nlev + 4) * nsteps
+        nbytes = members * sref_bytes(s.nlon, s.nlat, s.nlm, nlev) * nsteps
+        out[prec] = {"J_sum_K": round(J, 9), "forward_ms_per_step": round(1000 * t_fwd / nsteps, 3),
+                     "forward_sref_transforms_per_s": round(ntr / t_fwd, 0),
+                     "forward_transform_hbm_GBs": round(nbytes / t_fwd / 1e9, 1),
+                     "gradient_s": round(t_grad, 3), "gradient_over_forward": round(t_grad / t_fwd, 2),
+                     "peak_torch_memory_GB": round(torch.cuda.max_memory_allocated() / 1e9, 2),
+                     "device_memory_in_use_GB": round((total - free) / 1e9, 2)}
+        del st, adj
+        torch.cuda.empty_cache()
+    if "split" in finals and "fp64" in finals:
+        out["split_final_state_rel_err_vs_fp64"] = {
+            k: float(f"{rel_err_any(finals['split'][k], finals['fp64'][k]):.3e}") for k in finals["fp64"]}
+        out["split_gradient_rel_err_vs_fp64"] = {
+            k: float(f"{rel_err_any(grads['split'][k], grads['fp64'][k]):.3e}") for k in grads["fp64"]}
+    out["workload"] = (f"{members} x T{T}L{nlev} Held-Suarez members in one batched stepper, {nsteps} steps of "
+                       f"{dt:.0f} s (one day) + checkpointed gradient (every {every})")
+    return out
+
+
 def rel_err_any(a, ref):
     """max|a - ref| / max|ref| over a whole tensor (the reference's measure)."""
+    if a.is_complex():
+        a, ref = torch_real(a), torch_real(ref)
     a, ref = a.cpu().numpy(), ref.cpu().numpy()
     return float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
 
@@ -581,7 +694,9 @@ def main():
     ap.add_argument("--no-hs", action="store_true", help="skip the 1200-day Held-Suarez climate runs (~25 s)")
     ap.add_argument("--no-stress", action="store_true", help="skip the 32 x T106L30 stress plan")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--only", choices=["set", "ensemble", "step", "stress", "hs", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["set", "ensemble", "ensemble-step", "step", "stress", "stress-day", "hs",
+                                       "adjoint"],
+                    default=None,
                     help="profiling runs: only the T42L20 set (no ensemble, no FP64 forms), only the "
                          "ensemble, or only whole Held-Suarez dynamics steps")
     ap.add_argument("--path", choices=["auto", "tiled", "staged"], default="auto",
@@ -617,12 +732,19 @@ def main():
     if args.only == "hs":
         print(json.dumps({"held_suarez_T31L20_1200d": held_suarez_climate(torch)}))
         return
+    if args.only == "stress-day":
+        print(json.dumps({"stress_T106L30x32_day": stress_day_T106(torch)}))
+        return
     if args.only == "adjoint":
         print(json.dumps({"adjoint_T31L20_96": adjoint_T31(torch)}))
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble_config2": ensemble_config2(torch, args.precision, args.steps, args.warmup,
                                                                args.path)}))
+        return
+    if args.only == "ensemble-step":
+        print(json.dumps({"ensemble_config2_step": ensemble_config2_step(torch, args.precision, args.steps,
+                                                                         args.warmup)}))
         return
     sht = AerosSht(T)
     sht.set_path("staged" if args.path == "auto" else args.path)
@@ -758,6 +880,7 @@ def main():
         "kernels": kernels,
     }
     line["ensemble_config2"] = ensemble_config2(torch, args.precision, path=args.path)
+    line["ensemble_config2_step"] = ensemble_config2_step(torch, args.precision)
     line["held_suarez_step_T42L20"] = {
         "what": "one whole dynamics step: transform halves, column terms + HS forcing, semi-implicit "
                 "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
@@ -768,6 +891,7 @@ def main():
         line["held_suarez_T31L20_1200d"] = held_suarez_climate(torch)
     if not args.no_stress:
         line["stress_T106L30x32"] = stress_T106(torch)
+        line["stress_T106L30x32_day"] = stress_day_T106(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

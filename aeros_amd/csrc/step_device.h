@@ -76,14 +76,30 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
     double *PC = DBAR + n * NC;   // per column: pold, pnow, rp, lpnow, pstar
     const bool semi = S.flags & ST_SEMI, lin = S.flags & ST_LINEAR;
 
-    // 0. loads: every own level of the column, plus the column's ln p_s
+    // 0. loads: every own level of the column, plus the column's ln p_s; the vorticity
+    //    levels go to registers here so that no load waits at the end of the step
     double dn[NR], tn[NR];   // X^(n+1) of this thread's levels
+    double vo[NR], vt[NR], vc[NR], fv[NR], fd[NR];
+    const bool vload = live && !lin && (S.flags & ST_VOR);
+    const bool vfilt = vload && (S.flags & ST_RAW);
+    const bool fload = !lin && (S.flags & ST_DIFF);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int k = kl + r * AER_SI_KL;
+        vo[r] = vt[r] = vc[r] = 0.0;
+        fv[r] = fd[r] = 1.0;
         if (k >= n) break;
         const long x = k * ls + q;
         const int o = k * NC + c;
+        if (fload) {
+            fv[r] = S.fvt[k * (S.lmax + 1) + l];
+            fd[r] = S.fdiv[k * (S.lmax + 1) + l];
+        }
+        if (vload) {
+            vo[r] = S.ovor[x];
+            vt[r] = S.tvor[x];
+        }
+        if (vfilt) vc[r] = S.cvor[x];
         DOLD[o] = (live && !lin) ? S.odiv[x] : 0.0;
         DNOW[o] = live ? S.cdiv[x] : 0.0;
         TOLD[o] = (live && !lin) ? S.otemp[x] : 0.0;
@@ -105,9 +121,11 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double acc = S.pcoef[k] * PC[1 * NC + c];
+#pragma unroll 8
             for (int j = k; j < n; ++j) acc = acc + S.G[k * n + j] * TNOW[j * NC + c];
             const double ld = lam * acc;
             double at = 0.0;
+#pragma unroll 8
             for (int j = 0; j <= k; ++j) at = at + S.tau[k * n + j] * DNOW[j * NC + c];
             const int o = k * NC + c;
             LD[o] = ld;
@@ -141,9 +159,11 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double acc = S.pcoef[k] * PC[4 * NC + c];
+#pragma unroll 8
             for (int j = k; j < n; ++j) acc = acc + S.G[k * n + j] * TS[j * NC + c];
             const double ldstar = lam * acc;
             double w = 0.0;
+#pragma unroll 8
             for (int j = 0; j < n; ++j) w = w + S.W[k * n + j] * DOLD[j * NC + c];
             const int o = k * NC + c;
             RHS[o] = DOLD[o] - S.hh_old * lam * w + S.h * (RD[o] + ldstar - LD[o]);
@@ -156,6 +176,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double acc = 0.0;
+#pragma unroll 8
             for (int j = 0; j < n; ++j) acc = acc + Mi[k * n + j] * RHS[j * NC + c];
             dn[r] = acc;
             const int o = k * NC + c;
@@ -168,6 +189,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double acc = 0.0;
+#pragma unroll 8
             for (int j = 0; j <= k; ++j) acc = acc + S.tau[k * n + j] * DBAR[j * NC + c];
             const int o = k * NC + c;
             tn[r] = TOLD[o] + S.h * (RT[o] - acc - LT[o]);
@@ -204,15 +226,14 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
         const int o = k * NC + c;
         double nv = 0.0, ov = 0.0, cv = 0.0;
         if (vor) {
-            ov = S.ovor[x];
-            nv = ov + S.h * S.tvor[x];
+            ov = vo[r];
+            nv = ov + S.h * vt[r];
         }
         double nd = dn[r], nt = tn[r];
         if (diff) {
-            const double fv = S.fvt[k * (S.lmax + 1) + l], fd = S.fdiv[k * (S.lmax + 1) + l];
-            nv = fv * nv;
-            nd = fd * nd;
-            nt = fv * nt;
+            nv = fv[r] * nv;
+            nd = fd[r] * nd;
+            nt = fv[r] * nt;
         }
         if (raw) {
             double cd = DNOW[o], ct = TNOW[o];
@@ -225,7 +246,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             S.cdiv[x] = cd;
             S.ctemp[x] = ct;
             if (vor) {
-                cv = S.cvor[x];
+                cv = vc[r];
                 d = S.half_nu * (ov - 2.0 * cv + nv);
                 cv = cv + S.raw_a * d;
                 nv = nv - S.raw_b * d;
@@ -408,6 +429,7 @@ __global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
             ot[r] += tr;
             rt[r] += S.h * tr;
             double acc = 0.0;
+#pragma unroll 8
             for (int kk = k; kk < n; ++kk) acc = acc + S.tau[kk * n + k] * TRAW[kk * NC + c];
             const double dbar = -S.h * acc - S.h * S.nu[k] * praw;
             DNRT[o] = draw[r] + S.a1 * dbar;
@@ -422,6 +444,7 @@ __global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double acc = 0.0;
+#pragma unroll 8
             for (int kk = 0; kk < n; ++kk) acc = acc + Mi[kk * n + k] * DNRT[kk * NC + c];
             RHSB[k * NC + c] = acc;
         }
@@ -438,12 +461,14 @@ __global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
             if (k >= n) break;
             const int o = k * NC + c;
             double w = 0.0;
+#pragma unroll 8
             for (int kk = 0; kk < n; ++kk) w = w + S.W[kk * n + k] * RHSB[kk * NC + c];
             od[r] += RHSB[o] - S.hh_old * lam * w;
             rd[r] += S.h * RHSB[o];
             LDB[o] = -S.h * RHSB[o];
             // T* = T^old + astep (R_T - L_T): adjoint of T*
             double ts = 0.0;
+#pragma unroll 8
             for (int kk = 0; kk <= k; ++kk) ts = ts + S.G[kk * n + k] * RHSB[kk * NC + c];
             ts *= lam * S.h;
             ot[r] += ts;
@@ -465,9 +490,11 @@ __global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
             double tn = 0.0;
+#pragma unroll 8
             for (int kk = 0; kk <= k; ++kk) tn = tn + S.G[kk * n + k] * LDB[kk * NC + c];
             nt[r] += lam * tn;
             double dn = 0.0;
+#pragma unroll 8
             for (int kk = k; kk < n; ++kk) dn = dn + S.tau[kk * n + k] * LTB[kk * NC + c];
             nd[r] += -dn - S.nu[k] * blp;
         }

@@ -793,7 +793,12 @@ def main():
     ms_eager = time_steps(torch, ts.run_batched, args.steps, args.warmup)
     ms_calls = time_steps(torch, ts.run_calls, args.steps, args.warmup)
     graph = capture_graph(torch, sht, ts.run_batched)
-    ms = time_steps(torch, graph.replay, args.steps, args.warmup)
+    ms_graph = time_steps(torch, graph.replay, args.steps, args.warmup)
+    # the same four launches either way; `value` is the faster way of issuing them
+    ms, timing = ((ms_graph, "HIP graph replay of one set per step") if ms_graph <= ms_eager else
+                  (ms_eager, "eager launches of one set per step (4 kernels; faster than its HIP-graph "
+                             "replay on this runtime, which is reported beside it)"))
+    ms_fp64_best = min(ms_fp64, ms_fp64_eager)
     torch.cuda.synchronize()
     graph_errs = {k: rel_err(v, ref[k]) for k, v in ts.outputs().items()}
     assert max(graph_errs.values()) <= max(errs.values()) * 1.0001 + 1e-300, "graph replay differs from eager"
@@ -857,17 +862,18 @@ def main():
                    "precision": ("fp32-mfma; the (zeta_t, D_t) assembly job fp64 in the same launch"
                                  if args.precision == "fp32" else "fp64-mfma"),
                    "io": "fp64 in HBM"},
-        "timing": "HIP graph replay of one set per step (the kernels of the configuration below)",
+        "timing": timing,
         "kernel_path": args.path,
         "autotune": tuned,
         "autotune_fp64": tuned64,
         "eager_sets_per_s": round(1000.0 / ms_eager, 2),
+        "graph_sets_per_s": round(1000.0 / ms_graph, 2),
         "eager_4_calls_sets_per_s": round(1000.0 / ms_calls, 2),
         "fp64_batched_sets_per_s": round(1000.0 / ms_fp64, 2),
         "fp64_batched_eager_sets_per_s": round(1000.0 / ms_fp64_eager, 2),
         "fp64_plain_sets_per_s": round(1000.0 / ms_plain, 2),
         "speedup_vs_fp64_plain": round(ms_plain / ms, 2),
-        "speedup_vs_fp64_batched": round(ms_fp64 / ms, 3),
+        "speedup_vs_fp64_batched": round(ms_fp64_best / ms, 3),
         "max_rel_err_vs_fp64": max(errs.values()),
         "rel_err_vs_fp64_per_field": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "rel_err_all_fp32_per_field": (None if errs_all is None else

@@ -1941,7 +1941,7 @@ struct aer_semiimp {
     bool diff = false;
     double tau_diff = 0.0, tau_div = 0.0;
     int ndiff = 0;
-    DevBuf dG, dtau, dW, dnu, dpcoef, dMinv, dfvt, dfdiv, dtl, dtc;
+    DevBuf dG, dtau, dW, dnu, dpcoef, dMinv, dfvt, dfdiv, dtl;   // dtl: degree of each lm
 };
 
 namespace {
@@ -2053,8 +2053,7 @@ int launch_step(aer_semiimp *s, int flags, double eps, double raw_alpha,
     A.Minv = (const double *)s->dMinv.p;
     A.fvt = (const double *)s->dfvt.p;
     A.fdiv = (const double *)s->dfdiv.p;
-    A.task_l = (const int *)s->dtl.p;
-    A.task_c0 = (const int *)s->dtc.p;
+    A.l_of_lm = (const int *)s->dtl.p;
     A.ovor = ov; A.odiv = od; A.otemp = ot; A.olnps = op;
     A.cvor = cv; A.cdiv = cd; A.ctemp = ct; A.clnps = cp;
     A.tvor = tv; A.tdiv = td; A.ttemp = tt; A.tlnps = tp;
@@ -2119,13 +2118,15 @@ int aer_semiimp_create(aer_sht_t h, aer_vgrid_t vg, double t_ref, double ps_ref,
             for (int m = 0; m < n; ++m) w = w + s->G[k * n + m] * s->tau[m * n + j];
             s->W[k * n + j] = w;
         }
-    std::vector<int> tl, tc;
-    for (int l = 0; l <= s->lmax; ++l)
-        for (int c0 = 0; c0 < 2 * (l + 1); c0 += AER_SI_CB) {
-            tl.push_back(l);
-            tc.push_back(c0);
-        }
-    s->nblk = (int)tl.size();
+    // degree of each m-major spectral index; a block takes 16 consecutive real entries
+    std::vector<int> tl;
+    for (int m = 0; m <= s->lmax; ++m)
+        for (int l = m; l <= s->lmax; ++l) tl.push_back(l);
+    if ((int)tl.size() != s->h->nlm) {
+        aer_semiimp_destroy(s);
+        return fail(AER_EINVAL, "aer_semiimp_create: the step needs a triangular truncation (mmax = lmax, mres = 1)");
+    }
+    s->nblk = (int)((2L * s->h->nlm + AER_SI_CB - 1) / AER_SI_CB);
     int rc = 0;
     if (!rc) rc = upload(s->dG, s->G.data(), s->G.size() * sizeof(double));
     if (!rc) rc = upload(s->dtau, s->tau.data(), s->tau.size() * sizeof(double));
@@ -2133,7 +2134,6 @@ int aer_semiimp_create(aer_sht_t h, aer_vgrid_t vg, double t_ref, double ps_ref,
     if (!rc) rc = upload(s->dnu, s->nu.data(), s->nu.size() * sizeof(double));
     if (!rc) rc = upload(s->dpcoef, s->pcoef.data(), s->pcoef.size() * sizeof(double));
     if (!rc) rc = upload(s->dtl, tl.data(), tl.size() * sizeof(int));
-    if (!rc) rc = upload(s->dtc, tc.data(), tc.size() * sizeof(int));
     if (!rc) rc = si_build_step(s);
     if (rc) {
         aer_semiimp_destroy(s);
@@ -2188,7 +2188,7 @@ int aer_semiimp_gwspeed(aer_semiimp_t s, double *cmax)
 int aer_semiimp_destroy(aer_semiimp_t s)
 {
     if (!s) return AER_OK;
-    for (DevBuf *b : {&s->dG, &s->dtau, &s->dW, &s->dnu, &s->dpcoef, &s->dMinv, &s->dfvt, &s->dfdiv, &s->dtl, &s->dtc})
+    for (DevBuf *b : {&s->dG, &s->dtau, &s->dW, &s->dnu, &s->dpcoef, &s->dMinv, &s->dfvt, &s->dfdiv, &s->dtl})
         b->release();
     delete s;
     return AER_OK;
@@ -2305,8 +2305,7 @@ int aer_leapfrog_adj_batch_d(aer_semiimp_t s, int nmem, int flags, double eps_fi
     A.Minv = (const double *)s->dMinv.p;
     A.fvt = (const double *)s->dfvt.p;
     A.fdiv = (const double *)s->dfdiv.p;
-    A.task_l = (const int *)s->dtl.p;
-    A.task_c0 = (const int *)s->dtc.p;
+    A.l_of_lm = (const int *)s->dtl.p;
     A.bnvor = bn_vor; A.bndiv = bn_div; A.bntemp = bn_temp; A.bnlnps = bn_lnps;
     A.bcvor = bc_vor; A.bcdiv = bc_div; A.bctemp = bc_temp; A.bclnps = bc_lnps;
     A.govor = go_vor; A.godiv = go_div; A.gotemp = go_temp; A.golnps = go_lnps;

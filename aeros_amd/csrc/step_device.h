@@ -7,11 +7,13 @@
 //
 // The semi-implicit operators act on the levels of one spectral coefficient and
 // are real, so the real and imaginary parts are independent columns. A block
-// owns one degree l (its Helmholtz matrix) and 16 of that degree's 2(l+1) real
-// columns, with 16 level lanes per column: every stage is a small matrix-vector
-// product over LDS-resident columns. The Helmholtz solve applies the FP64
-// inverse of I + (alpha h)^2 lambda_l W, which the host builds from the same
-// pivoted LU factors the reference uses.
+// owns 16 consecutive real entries of a level row (8 consecutive lm, real and
+// imaginary parts), with 16 level lanes per column, so every level-row access of
+// the block is one contiguous 128-byte segment; each column applies its own
+// degree's operators. Every stage is a small matrix-vector product over
+// LDS-resident columns. The Helmholtz solve applies the FP64 inverse of
+// I + (alpha h)^2 lambda_l W, which the host builds from the same pivoted LU
+// factors the reference uses.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,7 +32,7 @@ struct StepArgs {
     const double *G, *tau, *W, *nu, *pcoef; // linear operators (row-major nlev x nlev, nlev)
     const double *Minv;                     // (lmax+1) x nlev x nlev
     const double *fvt, *fdiv;               // diffusion factors [nlev][lmax+1]
-    const int *task_l, *task_c0;            // block -> degree, first real column of the degree
+    const int *l_of_lm;                     // degree of each (m-major) spectral index
     const double *ovor, *odiv, *otemp, *olnps;   // X^(n-1)
     double *cvor, *cdiv, *ctemp, *clnps;         // X^n (the filter updates it in place)
     const double *tvor, *tdiv, *ttemp, *tlnps;   // R(X^n)
@@ -61,11 +63,9 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
     extern __shared__ double lds[];
     const int n = S.nlev;
     const int c = threadIdx.x % AER_SI_CB, kl = threadIdx.x / AER_SI_CB;
-    const int l = S.task_l[blockIdx.x], cg = S.task_c0[blockIdx.x] + c;
-    const bool live = cg < 2 * (l + 1);
-    const int m = live ? cg / 2 : 0, part = cg & 1;
-    const int lm = m * (2 * S.lmax + 3 - m) / 2 + (l - m);   // 0-based m-major index
-    const long q = 2L * lm + part;                           // real index in a level
+    const long q = (long)blockIdx.x * AER_SI_CB + c;         // real index in a level
+    const bool live = q < 2L * S.nlm;
+    const int l = S.l_of_lm[live ? (int)(q >> 1) : 0];
     const long ls = 2L * S.nlm;                              // level stride in doubles
     const double lam = (double)(l * (l + 1)) / (6.371e6 * 6.371e6);
     constexpr int NR = AER_SI_MAXLEV / AER_SI_KL;
@@ -318,13 +318,14 @@ namespace aer {
 
 // Adjoint of k_step (the transpose of the linear map (X^(n-1), X^n, R) -> (X^(n+1),
 // filtered X^n) that one leapfrog step applies at fixed operators). No reference
-// counterpart. Same block shape as k_step: one degree l, 16 real columns, 16 level
-// lanes; every stage is a transposed small matrix-vector product over LDS columns.
+// counterpart. Same block shape as k_step: 16 consecutive real entries of a level
+// row, 16 level lanes; every stage is a transposed small matrix-vector product over
+// LDS columns.
 struct StepAdjArgs {
     int nlev, lmax, nlm, flags;
     double h, a1, astep, hh_old, half_nu, raw_a, raw_b;
     const double *G, *tau, *W, *nu, *pcoef, *Minv, *fvt, *fdiv;
-    const int *task_l, *task_c0;
+    const int *l_of_lm;
     const double *bnvor, *bndiv, *bntemp, *bnlnps;   // adjoint of X^(n+1)
     const double *bcvor, *bcdiv, *bctemp, *bclnps;   // adjoint of the filtered X^n
     double *govor, *godiv, *gotemp, *golnps;         // adjoint of X^(n-1)
@@ -350,11 +351,9 @@ __global__ __launch_bounds__(256) void k_step_adj(const StepAdjArgs S0)
     extern __shared__ double lds[];
     const int n = S.nlev;
     const int c = threadIdx.x % AER_SI_CB, kl = threadIdx.x / AER_SI_CB;
-    const int l = S.task_l[blockIdx.x], cg = S.task_c0[blockIdx.x] + c;
-    const bool live = cg < 2 * (l + 1);
-    const int m = live ? cg / 2 : 0, part = cg & 1;
-    const int lm = m * (2 * S.lmax + 3 - m) / 2 + (l - m);
-    const long q = 2L * lm + part;
+    const long q = (long)blockIdx.x * AER_SI_CB + c;
+    const bool live = q < 2L * S.nlm;
+    const int l = S.l_of_lm[live ? (int)(q >> 1) : 0];
     const long ls = 2L * S.nlm;
     const double lam = (double)(l * (l + 1)) / (6.371e6 * 6.371e6);
     constexpr int NR = AER_SI_MAXLEV / AER_SI_KL;

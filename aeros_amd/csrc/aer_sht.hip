@@ -1832,7 +1832,16 @@ int column_launch(aer_sht *h, aer::ColumnArgs &C, int nmem, double *ae, double *
     C.ae = ae; C.an = an; C.ek = ek; C.dtdt = dtdt; C.dlnpsdt = dlnpsdt;
     const int nb = (C.ncol + AER_COL_CB - 1) / AER_COL_CB;
     const size_t lds = (size_t)(2 * (C.nlev + 1) + (7 * C.nlev + 1) * AER_COL_CB) * sizeof(double);
-    hipLaunchKernelGGL(aer::k_column_terms, dim3(nb, nmem), dim3(AER_COL_CB * AER_COL_KL), lds, h->stream, C);
+    // levels per thread: the smallest of 2, 3, 4, 8 that covers nlev over the block's level lanes
+    const dim3 grid(nb, nmem), block(AER_COL_CB * AER_COL_KL);
+    if (C.nlev <= 2 * AER_COL_KL)
+        hipLaunchKernelGGL(aer::k_column_terms<2>, grid, block, lds, h->stream, C);
+    else if (C.nlev <= 3 * AER_COL_KL)
+        hipLaunchKernelGGL(aer::k_column_terms<3>, grid, block, lds, h->stream, C);
+    else if (C.nlev <= 4 * AER_COL_KL)
+        hipLaunchKernelGGL(aer::k_column_terms<4>, grid, block, lds, h->stream, C);
+    else
+        hipLaunchKernelGGL(aer::k_column_terms<AER_COL_MAXLEV / AER_COL_KL>, grid, block, lds, h->stream, C);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }
@@ -2059,7 +2068,13 @@ int launch_step(aer_semiimp *s, int flags, double eps, double raw_alpha,
     A.tvor = tv; A.tdiv = td; A.ttemp = tt; A.tlnps = tp;
     A.nvor = nv; A.ndiv = nd; A.ntemp = nt; A.nlnps = np;
     const size_t lds = (size_t)(11 * s->nlev + 5) * AER_SI_CB * sizeof(double);
-    hipLaunchKernelGGL(aer::k_step, dim3(s->nblk, nmem), dim3(AER_SI_CB * AER_SI_KL), lds, s->h->stream, A);
+    // stage G, tau, W, nu, pcoef in LDS when the block still fits in 80 KB (two blocks per CU)
+    const size_t ops = (size_t)(3 * s->nlev * s->nlev + 2 * s->nlev) * sizeof(double);
+    const dim3 grid(s->nblk, nmem), block(AER_SI_CB * AER_SI_KL);
+    if (lds + ops <= 80 * 1024)
+        hipLaunchKernelGGL(aer::k_step<true>, grid, block, lds + ops, s->h->stream, A);
+    else
+        hipLaunchKernelGGL(aer::k_step<false>, grid, block, lds, s->h->stream, A);
     HIP_TRY(hipGetLastError());
     return AER_OK;
 }

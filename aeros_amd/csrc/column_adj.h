@@ -107,9 +107,9 @@ __global__ __launch_bounds__(64) void k_column_terms_adj(const ColumnAdjArgs X0)
             const double ph0 = A[k] + B[k] * ps, ph1 = A[k + 1] + B[k + 1] * ps;
             const double pf = 0.5 * (ph1 + ph0), dpf = 0.5 * (B[k] + B[k + 1]);
             const double sig = pf / ps, dsig = (dpf * ps - pf) / (ps * ps);
-            const double pfac = pf / kHsP0;
-            const double base = kHsTeq0 - kHsDTy * s2 - kHsDThz * log(pfac) * c2;
-            const double pk = pow(pfac, kKappa);
+            const double lpf = log(pf / kHsP0);   // as hs_teq in the forward kernel
+            const double base = kHsTeq0 - kHsDTy * s2 - kHsDThz * lpf * c2;
+            const double pk = exp(kKappa * lpf);
             const double teqr = base * pk;
             teq = teqr > kHsTmin ? teqr : kHsTmin;
             if (teqr > kHsTmin) dteq_dps = (-kHsDThz * c2 / pf * pk + base * kKappa * pk / pf) * dpf;

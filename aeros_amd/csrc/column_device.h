@@ -66,8 +66,14 @@ __device__ __forceinline__ ColumnArgs member_args(ColumnArgs C)
 struct LevelGeo {
     double ph0, ph1, dp, alpha, dlnp, cfac;
 };
-__device__ __forceinline__ LevelGeo level_geo(const ColumnArgs &C, const double *sA, const double *sB,
-                                              int k, double ps)
+// ln(p_{k+1/2} / p_{k-1/2}) of a level with a positive upper half level (else unused)
+__device__ __forceinline__ double level_lr(const double *sA, const double *sB, int k, double ps)
+{
+    const double ph0 = sA[k] + sB[k] * ps, ph1 = sA[k + 1] + sB[k + 1] * ps;
+    return ph0 > 0.0 ? log(ph1 / ph0) : 0.0;
+}
+__device__ __forceinline__ LevelGeo level_geo_lr(const ColumnArgs &C, const double *sA, const double *sB,
+                                                 int k, double ps, double lr)
 {
     LevelGeo g;
     g.ph0 = sA[k] + sB[k] * ps;
@@ -77,14 +83,30 @@ __device__ __forceinline__ LevelGeo level_geo(const ColumnArgs &C, const double 
         g.alpha = log(2.0);
         g.dlnp = (C.ztop && k == 0) ? 0.0 : kMV;
     } else {
-        const double lr = log(g.ph1 / g.ph0);
         g.alpha = 1.0 - (g.ph0 / g.dp) * lr;
         g.dlnp = lr;
     }
     g.cfac = (ps / g.dp) * (sB[k] * g.dlnp + (sB[k + 1] - sB[k]) * g.alpha);
     return g;
 }
+__device__ __forceinline__ LevelGeo level_geo(const ColumnArgs &C, const double *sA, const double *sB,
+                                              int k, double ps)
+{
+    return level_geo_lr(C, sA, sB, k, ps, level_lr(sA, sB, k, ps));
+}
 
+// Held-Suarez T_eq(phi, p) = max(200, (315 - dT_y sin^2 - dtheta_z ln(p/p0) cos^2) (p/p0)^kappa),
+// with (p/p0)^kappa as exp(kappa ln(p/p0)) so one logarithm serves both factors.
+__device__ __forceinline__ double hs_teq(double pf, double s2, double c2)
+{
+    const double lpf = log(pf / kHsP0);
+    const double teq = (kHsTeq0 - kHsDTy * s2 - kHsDThz * lpf * c2) * exp(kKappa * lpf);
+    return teq > kHsTmin ? teq : kHsTmin;
+}
+
+// NR: levels per thread (nlev <= NR * AER_COL_KL), chosen by the host so that the
+// per-thread level arrays hold only the levels that exist.
+template <int NR>
 __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
 {
     const ColumnArgs C = member_args(C0);
@@ -101,7 +123,6 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
     double *Cm = Tt + nlev * AER_COL_CB, *AT = Cm + nlev * AER_COL_CB, *H = AT + nlev * AER_COL_CB;
     double *S = H + nlev * AER_COL_CB;   // (nlev+1) x CB; reused for Phi after the scans
     double *PHI = AT;
-    constexpr int NR = AER_COL_MAXLEV / AER_COL_KL;
 
     // 1. every global load first: column 2-D fields, own levels' fields
     const double lnps = live ? C.lnps[cc] : 0.0;
@@ -130,11 +151,14 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
 
     // 2. per-level terms to LDS: winds, temperature, layer mass divergence C_k,
     //    the hydrostatic layer-point term alpha R T and the interface increment
+    double lr[NR];   // each own level's ln(p_{k+1/2}/p_{k-1/2}), kept for step 4
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int k = kl + r * AER_COL_KL;
+        lr[r] = 0.0;
         if (k >= nlev) break;
-        const LevelGeo g = level_geo(C, sA, sB, k, ps);
+        lr[r] = level_lr(sA, sB, k, ps);
+        const LevelGeo g = level_geo_lr(C, sA, sB, k, ps, lr[r]);
         const double G = ru[r] * gx + rv[r] * gy;
         const int o = k * AER_COL_CB + cl;
         U[o] = ru[r];
@@ -142,7 +166,7 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
         Tt[o] = rt[r];
         Cm[o] = g.dp * rd[r] + (sB[k + 1] - sB[k]) * ps * G;
         AT[o] = g.alpha * kRd * rt[r];
-        H[o] = g.ph0 > 0.0 ? kRd * rt[r] * log(g.ph1 / g.ph0) : 0.0;
+        H[o] = g.ph0 > 0.0 ? kRd * rt[r] * lr[r] : 0.0;
     }
     __syncthreads();
 
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
     for (int r = 0; r < NR; ++r) {
         const int k = kl + r * AER_COL_KL;
         if (k >= nlev || !live) break;
-        const LevelGeo g = level_geo(C, sA, sB, k, ps);
+        const LevelGeo g = level_geo_lr(C, sA, sB, k, ps, lr[r]);
         const int o = k * AER_COL_CB + cl;
         const double G = ru[r] * gx + rv[r] * gy;
         const double Sk = S[o], Ck = Cm[o];
@@ -199,9 +223,7 @@ __global__ __launch_bounds__(256) void k_column_terms(const ColumnArgs C0)
         if (C.hs) {
             const double pf = 0.5 * (g.ph1 + g.ph0);
             const double sig = pf / ps;
-            const double pfac = pf / kHsP0;
-            double teq = (kHsTeq0 - kHsDTy * s2 - kHsDThz * log(pfac) * c2) * pow(pfac, kKappa);
-            teq = teq > kHsTmin ? teq : kHsTmin;
+            double teq = hs_teq(pf, s2, c2);
             if (C.dteq) teq += C.dteq[(long)k * (C.ncol / C.nlon) + cc / C.nlon];
             double hf = (sig - kHsSigB) / (1.0 - kHsSigB);
             hf = hf > 0.0 ? hf : 0.0;

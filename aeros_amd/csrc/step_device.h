@@ -57,6 +57,9 @@ __device__ __forceinline__ StepArgs member_args(StepArgs S)
     return S;
 }
 
+// OPS: the l-independent operators G, tau, W, nu, pcoef are staged in LDS (the host
+// picks it when they fit beside the columns); otherwise they are read from memory.
+template <bool OPS>
 __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
 {
     const StepArgs S = member_args(S0);
@@ -74,7 +77,22 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
     double *RD = TNOW + n * NC, *RT = RD + n * NC, *LT = RT + n * NC, *LD = LT + n * NC;
     double *TS = LD + n * NC, *RHS = TS + n * NC, *DBAR = RHS + n * NC;
     double *PC = DBAR + n * NC;   // per column: pold, pnow, rp, lpnow, pstar
+    double *sops = PC + 5 * NC;   // with OPS: G, tau, W (n x n each), nu, pcoef
     const bool semi = S.flags & ST_SEMI, lin = S.flags & ST_LINEAR;
+    if constexpr (OPS) {
+        const int nn = n * n;
+        for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+            sops[i] = S.G[i];
+            sops[nn + i] = S.tau[i];
+            sops[2 * nn + i] = S.W[i];
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            sops[3 * nn + i] = S.nu[i];
+            sops[3 * nn + n + i] = S.pcoef[i];
+        }
+    }
+    const double *Gp = OPS ? sops : S.G, *Tp = OPS ? sops + n * n : S.tau, *Wp = OPS ? sops + 2 * n * n : S.W;
+    const double *NUp = OPS ? sops + 3 * n * n : S.nu, *PCp = OPS ? sops + 3 * n * n + n : S.pcoef;
 
     // 0. loads: every own level of the column, plus the column's ln p_s; the vorticity
     //    levels go to registers here so that no load waits at the end of the step
@@ -120,13 +138,13 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
         for (int r = 0; r < NR; ++r) {
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
-            double acc = S.pcoef[k] * PC[1 * NC + c];
+            double acc = PCp[k] * PC[1 * NC + c];
 #pragma unroll 8
-            for (int j = k; j < n; ++j) acc = acc + S.G[k * n + j] * TNOW[j * NC + c];
+            for (int j = k; j < n; ++j) acc = acc + Gp[k * n + j] * TNOW[j * NC + c];
             const double ld = lam * acc;
             double at = 0.0;
 #pragma unroll 8
-            for (int j = 0; j <= k; ++j) at = at + S.tau[k * n + j] * DNOW[j * NC + c];
+            for (int j = 0; j <= k; ++j) at = at + Tp[k * n + j] * DNOW[j * NC + c];
             const int o = k * NC + c;
             LD[o] = ld;
             LT[o] = -at;
@@ -134,7 +152,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
         }
         if (kl == 0) {
             double acc = 0.0;
-            for (int k = 0; k < n; ++k) acc = acc + S.nu[k] * DNOW[k * NC + c];
+            for (int k = 0; k < n; ++k) acc = acc + NUp[k] * DNOW[k * NC + c];
             const double lp = -acc;
             PC[3 * NC + c] = lp;
             PC[4 * NC + c] = PC[0 * NC + c] + S.astep * (PC[2 * NC + c] - lp);
@@ -158,13 +176,13 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
         for (int r = 0; r < NR; ++r) {
             const int k = kl + r * AER_SI_KL;
             if (k >= n) break;
-            double acc = S.pcoef[k] * PC[4 * NC + c];
+            double acc = PCp[k] * PC[4 * NC + c];
 #pragma unroll 8
-            for (int j = k; j < n; ++j) acc = acc + S.G[k * n + j] * TS[j * NC + c];
+            for (int j = k; j < n; ++j) acc = acc + Gp[k * n + j] * TS[j * NC + c];
             const double ldstar = lam * acc;
             double w = 0.0;
 #pragma unroll 8
-            for (int j = 0; j < n; ++j) w = w + S.W[k * n + j] * DOLD[j * NC + c];
+            for (int j = 0; j < n; ++j) w = w + Wp[k * n + j] * DOLD[j * NC + c];
             const int o = k * NC + c;
             RHS[o] = DOLD[o] - S.hh_old * lam * w + S.h * (RD[o] + ldstar - LD[o]);
         }
@@ -190,7 +208,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
             if (k >= n) break;
             double acc = 0.0;
 #pragma unroll 8
-            for (int j = 0; j <= k; ++j) acc = acc + S.tau[k * n + j] * DBAR[j * NC + c];
+            for (int j = 0; j <= k; ++j) acc = acc + Tp[k * n + j] * DBAR[j * NC + c];
             const int o = k * NC + c;
             tn[r] = TOLD[o] + S.h * (RT[o] - acc - LT[o]);
         }
@@ -210,7 +228,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs S0)
     if (kl == 0) {
         if (semi) {
             double acc = 0.0;
-            for (int k = 0; k < n; ++k) acc = acc + S.nu[k] * DBAR[k * NC + c];
+            for (int k = 0; k < n; ++k) acc = acc + NUp[k] * DBAR[k * NC + c];
             pn = PC[0 * NC + c] + S.h * (PC[2 * NC + c] + (-acc) - PC[3 * NC + c]);
         } else {
             pn = PC[0 * NC + c] + S.h * PC[2 * NC + c];

@@ -92,6 +92,54 @@ def run(trunc=31, nlev=20, dt=1800.0, days=1200.0, avg_start=200.0, sample_days=
     return summary(d, lat), d, info
 
 
+def run_ensemble(trunc=31, nlev=20, members=64, dt=1800.0, days=1200.0, avg_start=200.0, sample_days=1.0,
+                 precision="fp64", seed_offset=0, progress=None):
+    """The same protocol for an ensemble of `members` runs stepped together
+    (aeros_amd.ensemble.EnsembleStepper: one batched launch per stage, HIP-graph replay).
+    Member m starts from the isothermal state with the seed amplitude 1 + 1e-3 (m +
+    seed_offset), so members decorrelate within the spin-up. Every member's daily samples go
+    into one set of sums: the result is the ensemble-mean time-mean climate, whose sampling
+    noise is that of a single run over sqrt(members)."""
+    import torch
+
+    from .ensemble import EnsembleStepper
+    ens = EnsembleStepper([(trunc, nlev, members)], dt=dt, precision=precision)
+    s = ens.groups[0].st.sht
+    ens.autotune()
+    states = [initial_state(s, nlev, perturb=1.0 + 1.0e-3 * (m + seed_offset)) for m in range(members)]
+    ens.set_state([tuple(np.stack([st[i] for st in states]) for i in range(4))])
+    A, B = vgrid_init(nlev, stretch_a=1.0, stretch_r=2.0, sigma_t=0.0, p_top=1000.0, ps_ref=1.0e5)
+    stats = ZonalStats(s, nlev)
+    steps_per_sample = int(round(sample_days * DAY / dt))
+    nsample = int(round(days / sample_days))
+    t0 = time.time()
+    for n in range(1, nsample + 1):
+        for _ in range(steps_per_sample):
+            ens.step()
+            if ens.nstep == 2:
+                ens.capture()
+        day = n * sample_days
+        if day >= avg_start or (progress and n % max(1, nsample // progress) == 0):
+            now = ens.now(0)
+            g = s.tendency_grid_transforms(now["vor"], now["div"], now["temp"], now["lnps"])
+            if day >= avg_start:
+                for m in range(members):
+                    stats.accumulate({k: g[k][m] for k in ("u", "v", "t_g", "lnps_g")}, day)
+            if progress and n % max(1, nsample // progress) == 0:
+                umax = float(torch.sqrt(g["u"] ** 2 + g["v"] ** 2).max())
+                print(f"day {day:8.1f}  max|v| {umax:7.2f}  {time.time() - t0:7.1f} s", flush=True)
+                if not math.isfinite(umax) or umax > 300.0:
+                    raise RuntimeError(f"the ensemble went unstable at day {day}")
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    d = stats.finalize(A, B)
+    lat = np.degrees(np.arcsin(s.sinlat))
+    info = {"trunc": trunc, "nlev": nlev, "members": members, "dt": dt, "days": days, "avg_start": avg_start,
+            "precision": precision, "seed_offset": seed_offset, "steps": ens.nstep, "wall_s": round(wall, 2),
+            "ms_per_ensemble_step": round(1000.0 * wall / max(ens.nstep, 1), 4)}
+    return summary(d, lat), d, info
+
+
 def compare(da, db):
     """Differences of the time-mean zonal-mean [T] and [u] between two runs: largest and rms,
     as they are and after averaging the two hemispheres (the forcing is symmetric, so the
@@ -117,9 +165,32 @@ def main():
                     help="both: an FP64 run, an FP32-transform run and an FP64 twin whose seed differs "
                          "by 1e-12 relative (the chaotic twin measures the sampling noise of the window)")
     ap.add_argument("--out", default=None, help="write the summary (and the FP64-vs-FP32 comparison) as JSON")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="members per run (0: single runs); with --precision both: an FP64 ensemble, the same "
+                         "ensemble with FP32 transforms (the FP64 assembly job), and an independent FP64 ensemble")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
+    if args.ensemble > 0:
+        runs = ([("fp64", "fp64", 0), ("fp32", "fp32", 0), ("fp64_twin", "fp64", args.ensemble)]
+                if args.precision == "both" else [(args.precision, args.precision, 0)])
+        res = {}
+        for name, p, off in runs:
+            print(f"== Held-Suarez T{args.trunc}L{args.nlev} x {args.ensemble} members, {name}", flush=True)
+            sm, d, info = run_ensemble(args.trunc, args.nlev, args.ensemble, args.dt, args.days, args.avg_start,
+                                       precision=p, seed_offset=off, progress=6)
+            print(json.dumps({"summary": sm, "info": info}), flush=True)
+            res[name] = (sm, d, info)
+        out = {p: {"summary": res[p][0], "info": res[p][2]} for p in res}
+        if len(runs) == 3:
+            out["fp32_vs_fp64"] = compare(res["fp32"][1], res["fp64"][1])
+            out["fp64_twin_vs_fp64"] = compare(res["fp64_twin"][1], res["fp64"][1])
+            print(json.dumps({"fp32_vs_fp64": out["fp32_vs_fp64"], "fp64_twin_vs_fp64": out["fp64_twin_vs_fp64"]}),
+                  flush=True)
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(out, f, indent=1)
+        return
     runs = ([("fp64", "fp64", 1.0), ("fp32", "fp32", 1.0), ("fp64_twin", "fp64", 1.0 + 1e-12)]
             if args.precision == "both" else [(args.precision, args.precision, 1.0)])
     res = {}

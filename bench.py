@@ -549,18 +549,20 @@ def rel_err_any(a, ref):
     return float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
 
 
-def held_suarez_climate(torch, trunc=31, days=1200.0):
-    """The Held-Suarez protocol of drivers/held_suarez.f90 at T31L20: 1200 days, the last 1000
-    averaged, in FP64, with FP32 transforms, and an FP64 twin whose seed differs by 1e-12
-    relative (its difference from the FP64 run is the sampling noise of the window)."""
-    from aeros_amd.held_suarez import compare, run
-    out = {}
+def held_suarez_climate(torch, trunc=31, days=1200.0, members=32):
+    """The Held-Suarez protocol of drivers/held_suarez.f90 at T31L20 (1200 days, the last 1000
+    averaged) for ensembles of `members` runs stepped together: an FP64 ensemble, the same
+    ensemble with FP32 transforms (FP64 assembly job), and an independent FP64 ensemble whose
+    difference from the first is the sampling noise of the ensemble-mean climate."""
+    from aeros_amd.held_suarez import compare, run_ensemble
+    out = {"members": members}
     res = {}
-    for name, prec, amp in (("fp64", "fp64", 1.0), ("fp32", "fp32", 1.0), ("fp64_twin", "fp64", 1.0 + 1e-12)):
-        sm, d, info = run(trunc=trunc, nlev=20, dt=1800.0, days=days, avg_start=200.0, precision=prec, perturb=amp)
+    for name, prec, off in (("fp64", "fp64", 0), ("fp32", "fp32", 0), ("fp64_twin", "fp64", members)):
+        sm, d, info = run_ensemble(trunc=trunc, nlev=20, members=members, dt=1800.0, days=days, avg_start=200.0,
+                                   precision=prec, seed_offset=off)
         res[name] = d
         out[name] = {"summary": {k: round(v, 3) for k, v in sm.items()}, "wall_s": info["wall_s"],
-                     "ms_per_step": info["ms_per_step"]}
+                     "ms_per_ensemble_step": info["ms_per_ensemble_step"]}
     out["fp32_vs_fp64"] = {k: round(v, 4) for k, v in compare(res["fp32"], res["fp64"]).items()}
     out["fp64_twin_vs_fp64"] = {k: round(v, 4) for k, v in compare(res["fp64_twin"], res["fp64"]).items()}
     out["published_T31L20"] = {"u_max": 35.3, "u_max_lat": -42.7, "vt_max": 18.7, "uv_max": 74.4, "eke_max": 268.0,
@@ -691,7 +693,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-hs", action="store_true", help="skip the 1200-day Held-Suarez climate runs (~25 s)")
+    ap.add_argument("--no-hs", action="store_true",
+                    help="skip the 1200-day Held-Suarez climate ensembles (3 x 32 members, ~90 s)")
     ap.add_argument("--no-stress", action="store_true", help="skip the 32 x T106L30 stress plan")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--only", choices=["set", "ensemble", "ensemble-step", "step", "stress", "stress-day", "hs",
@@ -730,7 +733,7 @@ def main():
         print(json.dumps({"stress_T106L30x32": stress_T106(torch, steps=args.steps, warmup=args.warmup)}))
         return
     if args.only == "hs":
-        print(json.dumps({"held_suarez_T31L20_1200d": held_suarez_climate(torch)}))
+        print(json.dumps({"held_suarez_T31L20_1200d_x32": held_suarez_climate(torch)}))
         return
     if args.only == "stress-day":
         print(json.dumps({"stress_T106L30x32_day": stress_day_T106(torch)}))
@@ -894,7 +897,7 @@ def main():
         "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path)}
     line["adjoint_T31L20_96"] = adjoint_T31(torch)
     if not args.no_hs:
-        line["held_suarez_T31L20_1200d"] = held_suarez_climate(torch)
+        line["held_suarez_T31L20_1200d_x32"] = held_suarez_climate(torch)
     if not args.no_stress:
         line["stress_T106L30x32"] = stress_T106(torch)
         line["stress_T106L30x32_day"] = stress_day_T106(torch)

@@ -155,3 +155,20 @@ def test_ensemble_stepper_matches_group_steppers(torch):
             assert torch.equal(a, b), (gi, k)
             assert _rel(a, c) <= 1e-12, (gi, k, _rel(a, c))
         assert torch.allclose(eager.groups[gi].lnr_cum, ref.lnr_cum, rtol=0, atol=1e-13)
+
+
+def test_ensemble_held_suarez_statistics_match_single_run(torch):
+    """run_ensemble with one member reproduces held_suarez.run's zonal-mean statistics (same
+    start, same protocol; the ensemble path's fused plans change only rounding), and an
+    ensemble pools every member's samples."""
+    from aeros_amd.held_suarez import run, run_ensemble
+    kw = dict(trunc=21, nlev=8, dt=1800.0, days=4.0, avg_start=1.0)
+    _, d1, _ = run(precision="fp64", **kw)
+    _, de, info = run_ensemble(members=1, precision="fp64", **kw)
+    assert de["nsample"] == d1["nsample"] == 4
+    for k in ("u", "v", "temp", "vt", "eke"):
+        e = float(np.abs(de[k] - d1[k]).max() / max(np.abs(d1[k]).max(), 1e-300))
+        assert e <= 1e-9, (k, e)
+    _, d3, _ = run_ensemble(members=3, precision="fp64", **kw)
+    assert d3["nsample"] == 12
+    assert np.isfinite(d3["u"]).all() and np.isfinite(d3["temp"]).all()

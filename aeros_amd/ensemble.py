@@ -147,9 +147,16 @@ class EnsembleStepper:
     def set_state(self, states):
         """states: one (vor, div, temp, lnps) tuple per group, arrays with a leading member
         dimension ((nmem, nlev, nlm) and (nmem, nlm))."""
+        if len(states) != len(self.groups):
+            raise ValueError(f"expected {len(self.groups)} group states, got {len(states)}")
         for g, st in zip(self.groups, states):
+            if len(st) != 4:
+                raise ValueError("a group state is (vor, div, temp, lnps)")
             for k, a in zip(KEYS, st):
-                g.X[0][k].copy_(torch.as_tensor(a, dtype=torch.complex128))
+                t = torch.as_tensor(a, dtype=torch.complex128)
+                if tuple(t.shape) != tuple(g.X[0][k].shape):
+                    raise ValueError(f"{k}: expected shape {tuple(g.X[0][k].shape)}, got {tuple(t.shape)}")
+                g.X[0][k].copy_(t)
         self.nstep = 0   # captured graphs stay valid: the buffers do not move
 
     def now(self, gi):

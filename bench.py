@@ -218,6 +218,24 @@ def torch_real(z):
     return torch.view_as_real(z)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "transform_set_T42L20_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` in the measured set, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counters cannot be read inside a timed run; the summary
+    names its command). None when the kernel is not in it."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for row in f:
+                if row.startswith(f"| `aer::{kernel}<"):
+                    cells = [c.strip() for c in row.strip().strip("|").split("|")]
+                    return round(float(cells[6]) * 1e6), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    except (OSError, ValueError, IndexError):
+        pass
+    return None, None
+
+
 def time_steps(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -848,6 +866,7 @@ def main():
         roof = {"kernel": names[dom], "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None}
     roof["alg_per_launch"] = "see kernels[].alg_*_per_set / launches_per_set"
+    roof["traffic"], roof["traffic_source"] = measured_traffic(names[dom])
 
     line = {
         "metric": "SH transform sets/s at T42L20, mixed-prec vs FP64 on same GPU at stated rel err",

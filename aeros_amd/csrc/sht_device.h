@@ -527,6 +527,19 @@ __device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J,
     const bool cv = item < J.nitem;
     const bool has_ek = (J.kind == ANA_VECTOR) && J.gin2 != nullptr;
     const int gpi = has_ek ? 3 : 2;  // grid fields per vector item
+    // the epilogue's per-degree factors, loaded now so that no load waits after the MFMAs
+    double fav[4], fev[4];
+    {
+        const double *fa = C.lfac + ((J.kind == ANA_VECTOR && comp == 1) ? J.mode1 : J.mode0) * (C.T + 1);
+        const double *fe = C.lfac + J.mode2 * (C.T + 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kr = tile * 16 + P::row(kq, r);
+            const int l = m + par + 2 * kr;
+            fav[r] = kr < nk ? fa[l] : 0.0;
+            fev[r] = (has_ek && comp == 0 && kr < nk) ? fe[l] : 0.0;
+        }
+    }
     if constexpr (PREC == 2) {
         // split form: A = float4 slices, row exponents per (table, degree row)
         const float4 *__restrict__ qP = (const float4 *)C.tabPT;
@@ -637,16 +650,14 @@ __device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J,
     double *out = (J.kind == ANA_VECTOR && comp == 1) ? J.sout1 : J.sout0;
     if (!out) return;   // a vector job may leave its T output out (adjoint of the gradient)
     out += (long)item * J.sstride;
-    const double *fa = C.lfac + ((J.kind == ANA_VECTOR && comp == 1) ? J.mode1 : J.mode0) * (C.T + 1);
-    const double *fe = C.lfac + J.mode2 * (C.T + 1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int kr = tile * 16 + P::row(kq, r);
         if (kr >= nk) continue;
         const int l = m + par + 2 * kr;
         const int lm = lmb + (l - m);
-        double v = fa[l] * (double)acc[r];
-        if (has_ek && comp == 0) v += fe[l] * (double)acc2[r];
+        double v = fav[r] * (double)acc[r];
+        if (has_ek && comp == 0) v += fev[r] * (double)acc2[r];
         if (J.accumulate) out[2 * lm + part] += v;
         else out[2 * lm + part] = v;
     }
@@ -1041,7 +1052,11 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     // job fields read before the global stores below (kernel-argument memory is
     // then never re-read after a store that might alias it)
     const int M = C.T, nh = C.nh, nhp = C.nhp, ngf = J.ngf, adj = J.adj, nlat = C.nlat;
-    const double *wpair = C.wpair;
+    // the block's one or two latitude-pair weights, loaded before the FFT so that no
+    // global load waits after it
+    static_assert(ROWS <= 16, "a fixed FFT block spans at most two latitude pairs");
+    const double wp0 = p0 < nh ? C.wpair[p0] : 0.0;
+    const double wp1 = (ROWS > 8 && p0 + 1 < nh) ? C.wpair[p0 + 1] : 0.0;
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
     for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
         const int rp = idx % ROWS, m = idx / ROWS;
@@ -1049,7 +1064,8 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
         if (pair >= nhp) continue;
         T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
         if (pair < nh && g * 8 + gfl < ngf) {
-            const double w = adj ? ((nlat - 1 - pair == pair) ? 0.5 : 1.0) * (m > 0 ? 2.0 : 1.0) : wpair[pair];
+            const double w = adj ? ((nlat - 1 - pair == pair) ? 0.5 : 1.0) * (m > 0 ? 2.0 : 1.0)
+                                 : ((rp >> 3) ? wp1 : wp0);
             anal_unpack(res[pidx(rp * N + m)], res[pidx(rp * N + (m == 0 ? 0 : N - m))], T(w), o);
         } else {
             o[0] = o[1] = o[16] = o[17] = T(0);

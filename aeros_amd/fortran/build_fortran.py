@@ -10,8 +10,8 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "build")
-MODULES = ["aeros_amd_kinds.f90", "aer_sht_c.f90", "aeros_spectral.f90", "aeros_vordiv.f90"]
-TESTS = ["test_host", "test_spectral", "test_vordiv"]
+MODULES = ["aeros_amd_kinds.f90", "aer_sht_c.f90", "aer_dyn_c.f90", "aeros_spectral.f90", "aeros_vordiv.f90"]
+TESTS = ["test_host", "test_spectral", "test_vordiv", "test_step"]
 
 
 def _run(cmd, verbose):

@@ -686,6 +686,20 @@ def dynamics_step(torch, precision, steps=20, warmup=3, path="staged"):
     return round(ms, 4)
 
 
+def dynamics_step_graph(torch, precision, steps=200, warmup=10):
+    """The same T42L20 Held-Suarez step for one model through EnsembleStepper (one member):
+    the step's launches replayed from a HIP graph, no Python between the kernels. Returns
+    ms per step."""
+    from aeros_amd.ensemble import EnsembleStepper, rest_states
+    ens = EnsembleStepper([(T, NLEV, 1)], dt=1200.0, precision=precision)
+    ens.autotune()
+    ens.set_state(rest_states([(T, NLEV, 1)], seed_config=3))
+    ens.step()
+    ens.step()
+    ens.capture()
+    return round(time_steps(torch, ens.step, steps, warmup), 4)
+
+
 def cpu_baseline(budget_s=12.0):
     """The FP64 CPU restatement (oracle/) timed in the reference's mode-B harness shape:
     an OpenMP loop over levels, each unit one scalar synthesis + one analysis."""
@@ -745,6 +759,8 @@ def main():
     if args.only == "step":
         print(json.dumps({"held_suarez_step_ms": dynamics_step(torch, args.precision, args.steps, args.warmup,
                                                                args.path),
+                          "held_suarez_step_graph_ms": dynamics_step_graph(torch, args.precision, args.steps,
+                                                                           args.warmup),
                           "precision": args.precision, "kernel_path": args.path}))
         return
     if args.only == "stress":
@@ -913,7 +929,9 @@ def main():
         "what": "one whole dynamics step: transform halves, column terms + HS forcing, semi-implicit "
                 "leapfrog + del^6 diffusion + RAW filter (one kernel), FP64 mass fixer; eager launches",
         "ms_per_step_fp64": dynamics_step(torch, "fp64", path=args.path),
-        "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path)}
+        "ms_per_step_fp32_transforms": dynamics_step(torch, "fp32", path=args.path),
+        "graph_ms_per_step_fp64": dynamics_step_graph(torch, "fp64"),
+        "graph_ms_per_step_fp32_transforms": dynamics_step_graph(torch, "fp32")}
     line["adjoint_T31L20_96"] = adjoint_T31(torch)
     if not args.no_hs:
         line["held_suarez_T31L20_1200d_x32"] = held_suarez_climate(torch)

@@ -742,7 +742,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the transform set of one member is not split across "
-              "GPUs (ensemble members shard with no exchange; see DESIGN.md). Run with --gpus 1.",
+              "GPUs (ensemble members would shard across GPUs with no exchange). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     import torch

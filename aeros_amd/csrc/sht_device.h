@@ -499,11 +499,14 @@ __device__ __forceinline__ void leg_anal_body(const LaunchArgs &A, const Job &J,
     using Acc = typename P::Acc;
     const CfgDev &C = J.c;
     const int ntpp = ((C.T + 2) / 2 + 15) / 16;  // tiles per parity
+    // Task order: (parity, degree tile) fastest, then field group, then m. The tasks of one
+    // (group, m) read the same intermediate rows (both parities share each 128-byte pair
+    // record), so they run in one block, or in neighbouring blocks, and share L1/L2.
     int t = task - J.ltask0;
-    const int g = t % J.lgroups;
-    t /= J.lgroups;
     const int tt = t % (2 * ntpp);
-    const int m = t / (2 * ntpp);
+    t /= 2 * ntpp;
+    const int g = t % J.lgroups;
+    const int m = t / J.lgroups;
     const int par = tt / ntpp, tile = tt % ntpp;
     const int nk = (C.T - m - par >= 0) ? (C.T - m - par) / 2 + 1 : 0;
     if (tile * 16 >= nk) return;

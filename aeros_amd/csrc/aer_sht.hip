@@ -484,16 +484,33 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
     B = Table{};
     bool any64 = false, any32 = false, anysp = false;
     size_t cxmax = 0;
+    // FFT rows of the whole launch: few rows (one model's set) take the small blocks
+    long rows_all = 0;
+    for (int k = 0; k < njob; ++k) {
+        const JobSpec &s = specs[k];
+        if (s.nitem <= 0) continue;
+        const int ngf = (s.kind == aer::SYN_SCALAR || s.kind == aer::ANA_SCALAR) ? s.nitem
+                        : (s.kind == aer::SYN_VECTOR || s.kind == aer::SYN_GRAD) ? 2 * s.nitem
+                        : (s.gin2 ? 3 : 2) * s.nitem;
+        rows_all += (long)(ngf + 7) / 8 * 8 * hs[k]->nhp;
+    }
+    const bool small = rows_all <= aer::AER_FFT_SMALL_LAUNCH;
     for (int k = 0; k < njob; ++k) {
         const JobSpec &s = specs[k];
         if (s.nitem <= 0) continue;
         const aer_sht *h = hs[k];
-        const aer::CfgDev C = cfg_dev(h, s.prec == 1 ? AER_PREC_FP32 : (s.prec == 2 ? AER_PREC_SPLIT : AER_PREC_FP64));
+        aer::CfgDev C = cfg_dev(h, s.prec == 1 ? AER_PREC_FP32 : (s.prec == 2 ? AER_PREC_SPLIT : AER_PREC_FP64));
+        int frows = aer::fft_fixed_rows(h->nlon);
+        if (small && aer::fft_small_rows_ok(h->nlon)) {
+            frows = aer::AER_FFT_SMALL_ROWS;
+            C.fb = frows;   // fields per block: a slice of each group of 8
+            C.pb = 1;
+        }
         const size_t cxb = s.prec == 1 ? 2 * sizeof(float) : 2 * sizeof(double);
         (s.prec == 1 ? any32 : (s.prec == 2 ? anysp : any64)) = true;
         cxmax = std::max(cxmax, cxb);
         const int ntpp = ((C.T + 2) / 2 + 15) / 16;
-        const int nsub = 8 / h->fb;
+        const int nsub = 8 / C.fb;
         aer::Job J{};
         J.c = C;
         J.kind = s.kind;
@@ -524,7 +541,7 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         J.ltasks = synth ? J.lgroups * (C.T + 1) * C.npt : J.lgroups * (C.T + 1) * 2 * ntpp;
         B.ltask += (J.ltasks + 3) / 4 * 4;
         B.ftask += J.ngroup * nsub * (((synth ? C.nh : C.nhp) + C.pb - 1) / C.pb);
-        const size_t rn = (size_t)aer::fft_fixed_rows(h->nlon) * h->nlon;
+        const size_t rn = (size_t)frows * h->nlon;
         const size_t lds = aer::fft_fixed(h->nlon)
                                ? (2 * (rn + rn / 16 + 1) + h->nlon) * cxb
                                : (size_t)2 * h->fb * h->pb * h->nlon * cxb;

@@ -801,7 +801,27 @@ __host__ __device__ constexpr bool fft_fixed(int n)
 {
     return n == 64 || n == 96 || n == 128 || n == 160 || n == 192 || n == 256 || n == 320;
 }
+// Rows of a fixed-length FFT block: 16 at nlon 64, 8 otherwise. A launch with few rows in
+// all (one model's transform set) takes blocks of 4 rows at nlon 96 and 128 instead, so
+// the launch has twice the blocks and each a shorter chain (fft_small_rows).
 __host__ __device__ constexpr int fft_fixed_rows(int n) { return n == 64 ? 16 : 8; }
+__host__ __device__ constexpr bool fft_small_rows_ok(int n) { return n == 96 || n == 128; }
+constexpr int AER_FFT_SMALL_ROWS = 4;
+constexpr long AER_FFT_SMALL_LAUNCH = 16384;   // launches with at most this many rows take 4-row blocks
+
+// Row rp of a fixed-length FFT block. ROWS >= 8: the group's 8 fields x ROWS/8 latitude
+// pairs, field fastest; ROWS < 8: ROWS fields (the sub-th slice of the group) x one pair.
+template <int ROWS>
+__device__ __forceinline__ void fixed_row(int rp, int sub, int p0, int &gfl, int &pair)
+{
+    if constexpr (ROWS >= 8) {
+        gfl = rp & 7;
+        pair = p0 + (rp >> 3);
+    } else {
+        gfl = sub * ROWS + rp;
+        pair = p0;
+    }
+}
 
 // LDS index with one padding element per 16: the Stockham stores of a radix-R pass
 // stride R elements across lanes, which without padding hits a few banks only.
@@ -944,7 +964,8 @@ __device__ inline void anal_unpack(cx<T> zm, cx<T> zc, T w, T *__restrict__ out)
 // ---- synthesis FFT, compile-time length: block = 8 grid fields x ROWS/8 latitude pairs,
 // row rp = pair_local * 8 + field_local.
 template <class T, int N, int ROWS>
-__device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, const T *__restrict__ four)
+__device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, int g, int sub, int p0, cx<T> *smem,
+                                                const T *__restrict__ four)
 {
     constexpr int NT = AER_FFT_THREADS;
     constexpr int PL = padded_len<ROWS, N>();
@@ -963,7 +984,8 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
     for (int u = 0; u < NLD; ++u) {
         const int idx = threadIdx.x + u * NT;
         const int rp = idx % ROWS, m = idx / ROWS;
-        const int gfl = rp & 7, pair = p0 + (rp >> 3);
+        int gfl, pair;
+        fixed_row<ROWS>(rp, sub, p0, gfl, pair);
         const bool v = m <= M && pair < C.nh && g * 8 + gfl < J.ngf;
         const T *b = four + ((long)(v ? m : 0) * C.nhp + (v ? pair : 0)) * 32 + gfl * 2;
         er[u] = v ? b[0] : T(0);
@@ -1003,7 +1025,9 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
     const T2 *res = fft_block_fixed<+1, T, N, ROWS>(bufA, bufB, tw);
     for (int idx = threadIdx.x; idx < ROWS * N; idx += NT) {
         const int rp = idx / N, i = idx - rp * N;
-        const int gfl = rp & 7, pair = p0 + (rp >> 3), gf = g * 8 + gfl;
+        int gfl, pair;
+        fixed_row<ROWS>(rp, sub, p0, gfl, pair);
+        const int gf = g * 8 + gfl;
         if (pair >= C.nh || gf >= J.ngf) continue;
         double sc;
         double *out = syn_out(J, gf, sc);
@@ -1016,7 +1040,8 @@ __device__ __forceinline__ void fft_synth_fixed(const Job &J, const CfgDev &C, i
 }
 
 template <class T, int N, int ROWS>
-__device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, int g, int p0, cx<T> *smem, T *__restrict__ four)
+__device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, int g, int sub, int p0, cx<T> *smem,
+                                               T *__restrict__ four)
 {
     constexpr int NT = AER_FFT_THREADS;
     constexpr int PL = padded_len<ROWS, N>();
@@ -1031,7 +1056,9 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     for (int u = 0; u < NLD; ++u) {
         const int idx = threadIdx.x + u * NT;
         const int rp = idx / N, i = idx - rp * N;
-        const int gfl = rp & 7, pair = p0 + (rp >> 3), gf = g * 8 + gfl;
+        int gfl, pair;
+        fixed_row<ROWS>(rp, sub, p0, gfl, pair);
+        const int gf = g * 8 + gfl;
         const bool v = pair < C.nh && gf < J.ngf;
         double sc = 0.0;
         const double *in = ana_in(J, v ? gf : 0, sc);
@@ -1063,7 +1090,8 @@ __device__ __forceinline__ void fft_anal_fixed(const Job &J, const CfgDev &C, in
     const T2 *res = fft_block_fixed<-1, T, N, ROWS>(bufA, bufB, tw);
     for (int idx = threadIdx.x; idx < ROWS * (M + 1); idx += NT) {
         const int rp = idx % ROWS, m = idx / ROWS;
-        const int gfl = rp & 7, pair = p0 + (rp >> 3);
+        int gfl, pair;
+        fixed_row<ROWS>(rp, sub, p0, gfl, pair);
         if (pair >= nhp) continue;
         T *o = four + ((long)m * nhp + pair) * 32 + gfl * 2;
         if (pair < nh && g * 8 + gfl < ngf) {
@@ -1170,8 +1198,14 @@ __device__ __forceinline__ const Job &fft_task(const LaunchArgs &A, int task, in
 #define AER_FFT_DISPATCH(BODY, ...)                                                        \
     switch (C.nlon) {                                                                      \
         case 64: BODY##_fixed<T, 64, fft_fixed_rows(64)>(__VA_ARGS__); break;              \
-        case 96: BODY##_fixed<T, 96, fft_fixed_rows(96)>(__VA_ARGS__); break;              \
-        case 128: BODY##_fixed<T, 128, fft_fixed_rows(128)>(__VA_ARGS__); break;           \
+        case 96:                                                                           \
+            if (C.fb == AER_FFT_SMALL_ROWS) BODY##_fixed<T, 96, AER_FFT_SMALL_ROWS>(__VA_ARGS__); \
+            else BODY##_fixed<T, 96, fft_fixed_rows(96)>(__VA_ARGS__);                     \
+            break;                                                                         \
+        case 128:                                                                          \
+            if (C.fb == AER_FFT_SMALL_ROWS) BODY##_fixed<T, 128, AER_FFT_SMALL_ROWS>(__VA_ARGS__); \
+            else BODY##_fixed<T, 128, fft_fixed_rows(128)>(__VA_ARGS__);                   \
+            break;                                                                         \
         case 160: BODY##_fixed<T, 160, fft_fixed_rows(160)>(__VA_ARGS__); break;           \
         case 192: BODY##_fixed<T, 192, fft_fixed_rows(192)>(__VA_ARGS__); break;           \
         case 256: BODY##_fixed<T, 256, fft_fixed_rows(256)>(__VA_ARGS__); break;           \
@@ -1189,7 +1223,7 @@ __device__ __forceinline__ void fft_synth_body(const Job &J, int g, int sub, int
     cx<T> *smem = (cx<T> *)smem_raw;
     const T *four = (const T *)((const char *)four_base + J.four_off) + (long)g * (C.T + 1) * C.nhp * 32;
     if (fft_fixed(C.nlon)) {
-        AER_FFT_DISPATCH(fft_synth, J, C, g, p0, smem, four)
+        AER_FFT_DISPATCH(fft_synth, J, C, g, sub, p0, smem, four)
     } else if constexpr (GENERIC) {
         fft_synth_generic<T>(J, C, g, sub, p0, smem, four);
     }
@@ -1203,7 +1237,7 @@ __device__ __forceinline__ void fft_anal_body(const Job &J, int g, int sub, int 
     cx<T> *smem = (cx<T> *)smem_raw;
     T *four = (T *)((char *)four_base + J.four_off) + (long)g * (C.T + 1) * C.nhp * 32;
     if (fft_fixed(C.nlon)) {
-        AER_FFT_DISPATCH(fft_anal, J, C, g, p0, smem, four)
+        AER_FFT_DISPATCH(fft_anal, J, C, g, sub, p0, smem, four)
     } else if constexpr (GENERIC) {
         fft_anal_generic<T>(J, C, g, sub, p0, smem, four);
     }

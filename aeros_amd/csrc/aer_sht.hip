@@ -38,23 +38,32 @@ int fail(int code, const std::string &msg)
             return fail(AER_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
+// A device buffer that grows on demand. Growing never frees the previous allocation:
+// a HIP graph captured earlier holds that address in its kernel nodes (a handle's
+// Fourier intermediate, the mass grids), and a later, larger eager call on the same
+// handle must not pull it from under the graph's next replay. Superseded allocations
+// are kept until release() (the owner's destruction), so growth costs at most the sum
+// of the sizes the buffer went through.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    std::vector<void *> retired;
     int ensure(size_t n)
     {
         if (n <= bytes) return AER_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n);
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, n);
         if (e != hipSuccess) return fail(AER_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (p) retired.push_back(p);
+        p = q;
         bytes = n;
         return AER_OK;
     }
     void release()
     {
         if (p) (void)hipFree(p);
+        for (void *q : retired) (void)hipFree(q);
+        retired.clear();
         p = nullptr;
         bytes = 0;
     }

@@ -157,6 +157,40 @@ def test_ensemble_stepper_matches_group_steppers(torch):
         assert torch.allclose(eager.groups[gi].lnr_cum, ref.lnr_cum, rtol=0, atol=1e-13)
 
 
+def test_graph_replays_survive_a_larger_eager_call_on_the_handle(torch):
+    """The captured mass fixer synthesizes through the handle's Fourier intermediate (the
+    staged path, the default without autotuning). A larger eager transform on the same
+    handle afterwards (the climate driver's daily sampling) grows that intermediate; the
+    graph must keep a valid buffer, and its replays must still match eager steps exactly."""
+    from aeros_amd.ensemble import EnsembleStepper, rest_states
+    grp = [(21, 6, 3)]
+    states = rest_states(grp)
+    states[0][2][..., 1:] += (np.random.default_rng(7).standard_normal(states[0][2][..., 1:].shape) * 0.1)
+    nsteps = 9
+    eager = EnsembleStepper(grp, precision="fp64")
+    eager.set_state(states)
+    for _ in range(nsteps):
+        eager.step()
+    graph = EnsembleStepper(grp, precision="fp64")
+    graph.set_state(states)
+    s = graph.groups[0].st.sht
+    for n in range(nsteps):
+        graph.step()
+        if n == 1:
+            graph.capture()
+        if n == 4:
+            # 16 members x 20 levels: a far larger intermediate than the 3-field mass synthesis
+            z = torch.zeros((16, 20, s.nlm), dtype=torch.complex128, device="cuda")
+            zl = torch.zeros((16, s.nlm), dtype=torch.complex128, device="cuda")
+            g = s.tendency_grid_transforms(z, z, z, zl)
+            assert float(g["u"].abs().max()) == 0.0
+            del g
+    torch.cuda.synchronize()
+    for k in SKEYS:
+        assert torch.equal(eager.now(0)[k], graph.now(0)[k]), k
+    assert torch.equal(eager.groups[0].lnr_cum, graph.groups[0].lnr_cum)
+
+
 def test_ensemble_held_suarez_statistics_match_single_run(torch):
     """run_ensemble with one member reproduces held_suarez.run's zonal-mean statistics (same
     start, same protocol; the ensemble path's fused plans change only rounding), and an

@@ -193,8 +193,11 @@ int aer_tendency_spectral_batch_d(aer_sht_t cfg, int nmem, int nlev,
  * transforms of a model step (for example the tendency halves together with
  * passive-scalar transforms) into one launch pair, and to record that launch
  * pair into a HIP graph (no allocation happens inside after the first call of a
- * given shape). No reference counterpart: the reference loops over levels and
- * fields with one SHTns call each (src/dynamics/aeros_tendency.f90:409-419, 449-465). */
+ * given shape). A later, larger call on the same handle grows the handle's Fourier
+ * intermediate without freeing the previous one (kept until aer_sht_destroy), so a
+ * graph captured earlier stays valid. No reference counterpart: the reference loops
+ * over levels and fields with one SHTns call each (src/dynamics/aeros_tendency.f90:409-419,
+ * 449-465). */
 #define AER_JOB_SYNTH_SCALAR 0   /* in0 spectral -> out0 grid */
 #define AER_JOB_SYNTH_VECTOR 1   /* (in0 = S, in1 = T) -> (out0 = V_theta, out1 = V_phi) */
 #define AER_JOB_SYNTH_GRAD   2   /* in0 -> (out0 = d/dtheta, out1 = (1/sin) d/dphi) */
@@ -465,7 +468,9 @@ int aer_sht_sync(aer_sht_t cfg);
  * 1, 2, 3 the tiled path with m-band staging budgets of 16, 32 and 64 KiB. Each candidate
  * runs once untimed and reps times timed; synchronous; the jobs' outputs are overwritten.
  * times_ms (optional) receives the mean milliseconds per candidate (a plan: synthesis
- * candidates, then analysis candidates), choice (optional) the kept index (a plan: two). */
+ * candidates, then analysis candidates), choice (optional) the kept index (a plan: two).
+ * aer_plan_autotune rebuilds the plan's device job tables in place: a HIP graph that
+ * recorded aer_plan_run of this plan must be captured again afterwards. */
 #define AER_TUNE_NCAND 4
 int aer_autotune_jobs(aer_sht_t cfg, int njob, const aer_job *jobs, int reps, double *times_ms, int *choice);
 int aer_plan_autotune(aer_plan_t plan, void *stream, int reps, double *times_ms, int *choice);

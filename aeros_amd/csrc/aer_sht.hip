@@ -42,8 +42,9 @@ int fail(int code, const std::string &msg)
 // a HIP graph captured earlier holds that address in its kernel nodes (a handle's
 // Fourier intermediate, the mass grids), and a later, larger eager call on the same
 // handle must not pull it from under the graph's next replay. Superseded allocations
-// are kept until release() (the owner's destruction), so growth costs at most the sum
-// of the sizes the buffer went through.
+// are kept until release() (the owner's destruction). A regrowth takes at least 1.5x
+// the current size, so the kept allocations sum to at most twice the live one however
+// the requests creep upwards.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -51,12 +52,13 @@ struct DevBuf {
     int ensure(size_t n)
     {
         if (n <= bytes) return AER_OK;
+        const size_t want = bytes ? std::max(n, bytes + bytes / 2) : n;
         void *q = nullptr;
-        hipError_t e = hipMalloc(&q, n);
+        hipError_t e = hipMalloc(&q, want);
         if (e != hipSuccess) return fail(AER_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
         if (p) retired.push_back(p);
         p = q;
-        bytes = n;
+        bytes = want;
         return AER_OK;
     }
     void release()

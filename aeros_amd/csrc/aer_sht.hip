@@ -488,7 +488,24 @@ struct Table {
     size_t cxmax = 16;      // largest complex element size among the jobs
     bool tiled = false;     // tiled path (fused synthesis; 4x4-pair analysis FFT + per-m-pair Legendre)
     size_t lds_leg = 0;     // tiled analysis Legendre kernel's dynamic LDS
+    bool overflow = false;  // a task count left the 32-bit range (the launch is refused)
 };
+
+// Fields of one job: 3 grid fields per item (a vector analysis with its Laplacian term)
+// stay within 32 bits, as do the tables' task counts at any size that fits in memory.
+constexpr int kMaxFields = 1 << 26;
+
+// Adds n tasks to a 32-bit launch total; flags the table when the total (and the grid
+// rounded up to whole XCD rounds) would leave the int range.
+inline void add_tasks(Table &B, int &total, long n)
+{
+    const long t = (long)total + n;
+    if (n < 0 || t > 0x7fffffffL - 64) {
+        B.overflow = true;
+        return;
+    }
+    total = (int)t;
+}
 
 void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth, Table &B)
 {
@@ -549,9 +566,10 @@ void build_table(aer_sht *const *hs, const JobSpec *specs, int njob, bool synth,
         B.four_bytes += ((size_t)J.ngroup * (C.T + 1) * C.nhp * 32 * (cxb / 2) + 255) / 256 * 256;
         J.ltask0 = B.ltask;
         J.ftask0 = B.ftask;
-        J.ltasks = synth ? J.lgroups * (C.T + 1) * C.npt : J.lgroups * (C.T + 1) * 2 * ntpp;
-        B.ltask += (J.ltasks + 3) / 4 * 4;
-        B.ftask += J.ngroup * nsub * (((synth ? C.nh : C.nhp) + C.pb - 1) / C.pb);
+        const long lt = synth ? (long)J.lgroups * (C.T + 1) * C.npt : (long)J.lgroups * (C.T + 1) * 2 * ntpp;
+        J.ltasks = lt > 0x7fffffffL ? 0 : (int)lt;
+        add_tasks(B, B.ltask, (lt + 3) / 4 * 4);
+        add_tasks(B, B.ftask, (long)J.ngroup * nsub * (((synth ? C.nh : C.nhp) + C.pb - 1) / C.pb));
         const size_t rn = (size_t)frows * h->nlon;
         const size_t lds = aer::fft_fixed(h->nlon)
                                ? (2 * (rn + rn / 16 + 1) + h->nlon) * cxb
@@ -606,9 +624,10 @@ bool build_table_tiled(aer_sht *const *hs, const JobSpec *specs, int njob, bool 
         J.adj = s.adj;
         const int ntile = (C.nh + 3) / 4, ng4 = (J.ngf + 3) / 4;
         if (synth) {
+            const long lt = (long)ng4 * ntile;
             J.ltask0 = B.ltask;
-            J.ltasks = ng4 * ntile;
-            B.ltask += J.ltasks;
+            J.ltasks = lt > 0x7fffffffL ? 0 : (int)lt;
+            add_tasks(B, B.ltask, lt);
             // staging band: every coefficient when that fits beside the FFT buffers' LDS,
             // else bands of whole orders of at most chl coefficients (chl >= T+1)
             const int fft = aer::fz_fftbytes(h->nlon, csz);
@@ -621,14 +640,15 @@ bool build_table_tiled(aer_sht *const *hs, const JobSpec *specs, int njob, bool 
             B.lds = std::max(B.lds, (size_t)(aer::fz_fbytes(C.T, csz) + std::max(fft, aer::fz_stage_bytes(J.chl, csz))));
         } else {
             J.ftask0 = B.ftask;
-            B.ftask += ng4 * ntile;
+            add_tasks(B, B.ftask, (long)ng4 * ntile);
             J.four_off = (long)B.four_bytes;
             B.four_bytes += ((size_t)aer::fz_four_elems(J.ngf, C.T, C.nh) * csz + 255) / 256 * 256;
             const bool scalar = s.kind == aer::ANA_SCALAR;
             const int ncg = scalar ? (s.nitem + 3) / 4 : (s.nitem + 1) / 2;
+            const long lt = (long)ncg * ((C.T + 2) / 2);
             J.ltask0 = B.ltask;
-            J.ltasks = ncg * ((C.T + 2) / 2);
-            B.ltask += J.ltasks;
+            J.ltasks = lt > 0x7fffffffL ? 0 : (int)lt;
+            add_tasks(B, B.ltask, lt);
             B.lds = std::max(B.lds, (size_t)aer::fz_fftbytes(h->nlon, csz));
             const int nlgf = scalar ? 4 : 2 * (s.gin2 ? 3 : 2);
             B.lds_leg = std::max(B.lds_leg, (size_t)aer::la_lds(nlgf, C.nh, csz));
@@ -763,6 +783,7 @@ int run_jobs(aer_sht *h, JobSpec *specs, int njob, bool synth)
     }
     Table B;
     if (!build_table_tiled(hs, specs, njob, synth, B)) build_table(hs, specs, njob, synth, B);
+    if (B.overflow) return fail(AER_EINVAL, "launch too large: its task count leaves 32 bits; split the fields");
     if (B.jobs.empty()) return AER_OK;
     int rc = h->four.ensure(B.four_bytes);
     if (rc) return rc;
@@ -810,6 +831,7 @@ int check_batch(const aer_sht *h, int nfield, long qstride, long gstride)
     if (!h) return fail(AER_EINVAL, "null handle");
     if (!h->grid_set) return fail(AER_ESTATE, "transform before aer_sht_set_grid");
     if (nfield < 0) return fail(AER_EINVAL, "nfield < 0");
+    if (nfield > kMaxFields) return fail(AER_EINVAL, "nfield above 2^26 in one call");
     if (qstride < dense_s(h)) return fail(AER_EINVAL, "spectral stride smaller than 2*nlm");
     if (gstride < dense_g(h)) return fail(AER_EINVAL, "grid stride smaller than nlon*nlat");
     return AER_OK;
@@ -1154,6 +1176,7 @@ int job_to_spec(const aer_sht *h, const aer_job &j, JobSpec &s)
         return fail(AER_EINVAL, "job: unknown kind");
     const bool synth = j.kind <= AER_JOB_SYNTH_GRAD;
     if (j.nfield < 0) return fail(AER_EINVAL, "job: nfield < 0");
+    if (j.nfield > kMaxFields) return fail(AER_EINVAL, "job: nfield above 2^26");
     for (int md : {j.mode0, j.mode1, j.mode2})
         if (md < 0 || md >= AER_NMODE) return fail(AER_EINVAL, "job: unknown mode");
     const bool vec_in = j.kind == AER_JOB_SYNTH_VECTOR || j.kind == AER_JOB_ANAL_VECTOR;
@@ -1209,6 +1232,8 @@ int plan_build(aer_plan *p)
         build_table(hsy.data(), ssy.data(), (int)ssy.size(), true, p->syn);
     if (!build_table_tiled(han.data(), san.data(), (int)san.size(), false, p->ana))
         build_table(han.data(), san.data(), (int)san.size(), false, p->ana);
+    if (p->syn.overflow || p->ana.overflow)
+        return fail(AER_EINVAL, "plan too large: a launch's task count leaves 32 bits; split the jobs");
     int rc;
     if (!p->syn.jobs.empty() && (rc = upload(p->djob_syn, p->syn.jobs.data(), p->syn.jobs.size() * sizeof(aer::Job))))
         return rc;

@@ -154,3 +154,10 @@ def test_plan_validation(torch):
         L.check(L.lib().aer_plan_create(2, cf, (L.AerJob * 2)(j, bad), ctypes.byref(plan)))
     with pytest.raises(L.AerError):   # empty
         L.check(L.lib().aer_plan_create(0, cf, (L.AerJob * 2)(j, j), ctypes.byref(plan)))
+    # a field count beyond 2^26 is refused on the host, before any table or launch
+    huge = L.AerJob(L.AER_JOB_SYNTH_SCALAR, (1 << 26) + 1, x.data_ptr(), 0, 0, g.data_ptr(), 0, 0, 0, 0, 0, 0,
+                    0.0, 0.0, 0)
+    with pytest.raises(L.AerError, match="2\\^26"):
+        L.check(L.lib().aer_plan_create(2, cf, (L.AerJob * 2)(j, huge), ctypes.byref(plan)))
+    with pytest.raises(L.AerError, match="2\\^26"):
+        L.check(L.lib().aer_run_jobs_d(a._h, 1, (L.AerJob * 1)(huge)))

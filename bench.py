@@ -711,12 +711,18 @@ def cpu_baseline(budget_s=12.0):
     t_unit = o.bench_set(q, 1, threads)
     nrep = max(1, min(200, int(budget_s / max(t_unit * NLEV, 1e-6))))
     t_unit = o.bench_set(q, nrep, threads)
+    # one thread as well (the reference quotes per-core figures), on a quarter of the budget
+    t1 = o.bench_set(q, 1, 1)
+    nrep1 = max(1, min(50, int(0.25 * budget_s / max(t1 * NLEV, 1e-6))))
+    t1 = o.bench_set(q, nrep1, 1)
     units_per_set = (13 * NLEV + 4) / 2.0   # 264 scalar-equivalents = 132 (synthesis + analysis) units
     return {"value": 1.0 / (t_unit * units_per_set), "unit": "sets/s", "cores": threads,
             "kind": "port",
+            "value_1thread": 1.0 / (t1 * units_per_set),
             "sample": f"T42 FP64 oracle (dense Gauss-Legendre + own mixed-radix FFT), {NLEV} levels x "
                       f"{nrep} reps of scalar synthesis+analysis, OpenMP over levels; "
-                      f"{t_unit * 1e6 / 2:.1f} us per scalar-equivalent transform; sets/s = 1/(132 units)"}
+                      f"{t_unit * 1e6 / 2:.1f} us per scalar-equivalent transform; sets/s = 1/(132 units); "
+                      f"1 thread: {nrep1} reps, {t1 * 1e6 / 2:.1f} us per scalar-equivalent transform"}
 
 
 def main():

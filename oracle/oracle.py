@@ -59,6 +59,17 @@ def lib():
         L.orc_tendency_spectral.argtypes = [cp, ctypes.c_int] + [_dp] * 9
         L.orc_bench_set.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.orc_bench_set.restype = ctypes.c_double
+        L.orc_port_create.restype = ctypes.c_void_p
+        L.orc_port_create.argtypes = [cp]
+        L.orc_port_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_port_work_doubles.restype = ctypes.c_size_t
+        L.orc_port_work_doubles.argtypes = [ctypes.c_void_p]
+        L.orc_port_synth.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_port_anal.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_port_cfft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_port_cfft.restype = ctypes.c_int
+        L.orc_port_bench_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_port_bench_set.restype = ctypes.c_double
         d = ctypes.c_double
         L.orc_vgrid_init.argtypes = [ctypes.c_int, d, d, d, d, d, _dp, _dp]
         L.orc_vgrid_init.restype = ctypes.c_int
@@ -267,6 +278,54 @@ class Oracle:
         so = np.empty_like(spec_in)
         return lib().orc_bench_set(self._c, nlev, nrep, nthreads, spec_in.ctypes.data_as(_dp),
                                    _p(gw), so.ctypes.data_as(_dp))
+
+
+class Port:
+    """The oracle's scalar transforms arranged for CPU speed (oracle/cpu_port.c): what the
+    CPU baseline times. Same API as Oracle.synth / Oracle.anal."""
+
+    def __init__(self, oracle):
+        self._o = oracle   # keeps the tables' owner alive
+        self._p = lib().orc_port_create(oracle._c)
+        if not self._p:
+            raise ValueError("the CPU port needs an even nlat and nlon/2 with factors 2, 3, 5 only")
+        self.nlat, self.nlon, self.nlm = oracle.nlat, oracle.nlon, oracle.nlm
+        self._work = np.empty(lib().orc_port_work_doubles(self._p))
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.orc_port_destroy(self._p)
+            self._p = None
+
+    def synth(self, q):
+        q = np.ascontiguousarray(q, dtype=np.complex128)
+        g = np.empty((self.nlat, self.nlon))
+        lib().orc_port_synth(self._p, q.ctypes.data_as(_dp), _p(g), _p(self._work))
+        return g
+
+    def anal(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        q = np.empty(self.nlm, dtype=np.complex128)
+        lib().orc_port_anal(self._p, _p(g), q.ctypes.data_as(_dp), _p(self._work))
+        return q
+
+    def bench_set(self, spec_in, nrep, nthreads):
+        """Seconds per (synthesis + analysis) unit over an OpenMP loop of the rows of spec_in."""
+        spec_in = np.ascontiguousarray(spec_in, dtype=np.complex128)
+        nunit = spec_in.shape[0]
+        gw = np.empty((nunit, self.nlat, self.nlon))
+        so = np.empty_like(spec_in)
+        return lib().orc_port_bench_set(self._p, nunit, nrep, nthreads, spec_in.ctypes.data_as(_dp),
+                                        _p(gw), so.ctypes.data_as(_dp))
+
+
+def port_cfft(x, sign):
+    """Complex FFT of the CPU port: sum_j x_j e^{sign 2 pi i jk/n}."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(x)
+    if lib().orc_port_cfft(x.size, sign, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp)):
+        raise ValueError("length with a prime factor other than 2, 3, 5")
+    return y
 
 
 # ------------------------------------------------------------- column physics

@@ -13,7 +13,7 @@
  * ~1e-60 at T42), no hemispheric folding, the Fourier step is an own
  * mixed-radix complex FFT (or an exact O(N^2) DFT when fft_direct is set).
  *
- * What pins each function (tests/test_oracle.py, tests/test_oracle_dynamics.py):
+ * What pins each function (tests/test_oracle.py, tests/test_oracle_column.py):
  *   orc_grid_size, orc_lm     the reference's own table (tests/test_grid.f90:25-37,
  *                             tests/test_spectral.f90:45-68) and a brute-force
  *                             enumeration of (l, m)
@@ -33,8 +33,12 @@
  *   orc_surface_integral      4 pi a^2 and int Y_10 = 0 (tests/test_spectral.f90:89-104)
  *   vordiv / gradient         solid body, pure divergence, l = 0 discarded
  *                             (tests/test_vordiv.f90:59-289)
- *   tendency transforms       composition identities against the pinned pieces
- *                             and a planted-mistake check (test_oracle_dynamics.py)
+ *   tendency transforms       the rest-state balance and the global-mass identity
+ *                             through them, and a planted-mistake check
+ *                             (test_oracle_column.py)
+ *   orc_port_* (cpu_port.c)   the CPU-baseline arrangement: its FFT against numpy.fft,
+ *                             its synthesis / analysis against orc_synth / orc_anal
+ *                             at T5-T106 (tests/test_oracle.py)
  */
 #define _POSIX_C_SOURCE 199309L
 #include "sht_oracle.h"

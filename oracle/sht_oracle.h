@@ -16,6 +16,8 @@
 #ifndef AEROS_SHT_ORACLE_H
 #define AEROS_SHT_ORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -77,6 +79,20 @@ void   orc_tendency_spectral(const orc_cfg *c, int nlev,
 /* Mode-B timing harness: an OpenMP loop over independent (field, level) units. */
 double orc_bench_set(const orc_cfg *c, int nlev, int nrep, int nthreads,
                      const double *spec_in, double *grid_work, double *spec_out);
+
+/* The same scalar transforms arranged for CPU speed (cpu_port.c): hemispheric folding,
+ * parity-split tables, a half-length Stockham real FFT. Timed as the CPU baseline and
+ * checked against orc_synth / orc_anal. Needs an even nlat and nlon/2 of factors 2, 3, 5
+ * (NULL otherwise). work: orc_port_work_doubles() doubles per concurrent call. */
+typedef struct orc_port orc_port;
+orc_port *orc_port_create(const orc_cfg *c);
+void   orc_port_destroy(orc_port *p);
+size_t orc_port_work_doubles(const orc_port *p);
+void   orc_port_synth(const orc_port *p, const double *qlm, double *grid, double *work);
+void   orc_port_anal(const orc_port *p, const double *grid, double *qlm, double *work);
+int    orc_port_cfft(int n, int sign, const double *x, double *y);
+double orc_port_bench_set(const orc_port *p, int nunit, int nrep, int nthreads,
+                          const double *spec_in, double *grid_work, double *spec_out);
 
 #ifdef __cplusplus
 }

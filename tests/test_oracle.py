@@ -294,3 +294,42 @@ def test_planted_mistakes_are_caught():
     # weights not applied in the analysis: compare the unweighted sum
     bad = o.anal(z["anal_grid"] * 2.0 / o.w[:, None] / o.nlat)
     assert rel(bad, z["anal_spec"]) > 1e-3
+
+
+# ------------------------------------------------- the CPU-baseline arrangement
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 12, 20, 45, 48, 60, 64, 80, 96, 160, 240])
+def test_port_fft_against_numpy(n):
+    """The CPU port's Stockham FFT (radix 4, 2, 3, 5 passes) against numpy.fft, both signs."""
+    from oracle.oracle import port_cfft
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel(port_cfft(x, -1), np.fft.fft(x)) < 1e-14
+    assert rel(port_cfft(x, +1), np.fft.ifft(x) * n) < 1e-14
+    with pytest.raises(ValueError):
+        port_cfft(np.ones(7, complex), -1)
+
+
+@pytest.mark.parametrize("T", [5, 21, 31, 42, 106])
+def test_port_transforms_match_the_oracle(T):
+    """The CPU port (hemispheric folding, parity-split tables, half-length real FFT) is
+    the oracle's algorithm rearranged: synthesis and analysis agree with orc_synth /
+    orc_anal to rounding, including the m = 0 imaginary part that synthesis drops."""
+    from oracle.oracle import Port
+    o = Oracle(T)
+    p = Port(o)
+    rng = np.random.default_rng(T)
+    q = rng.standard_normal(o.nlm) + 1j * rng.standard_normal(o.nlm)   # m = 0 imaginary parts not zeroed
+    g0 = o.synth(q)
+    assert rel(p.synth(q), g0) < 1e-14
+    assert rel(p.anal(g0), o.anal(g0)) < 1e-14
+    # a grid that is not band-limited: the projections still agree
+    g = rng.standard_normal((o.nlat, o.nlon))
+    assert rel(p.anal(g), o.anal(g)) < 1e-13
+
+
+def test_port_rejects_grids_it_cannot_fold():
+    """An odd latitude count has no pairing of every row; the port refuses it."""
+    from oracle.oracle import Port
+    with pytest.raises(ValueError):
+        Port(Oracle(21, nlon=64, nlat=33))

@@ -61,6 +61,14 @@ struct DevBuf {
         bytes = want;
         return AER_OK;
     }
+    // Retires the current allocation (kept until release()), so the next ensure()
+    // allocates afresh: for contents that a captured graph must keep seeing unchanged.
+    void renew()
+    {
+        if (p) retired.push_back(p);
+        p = nullptr;
+        bytes = 0;
+    }
     void release()
     {
         if (p) (void)hipFree(p);
@@ -1234,6 +1242,11 @@ int plan_build(aer_plan *p)
         build_table(han.data(), san.data(), (int)san.size(), false, p->ana);
     if (p->syn.overflow || p->ana.overflow)
         return fail(AER_EINVAL, "plan too large: a launch's task count leaves 32 bits; split the jobs");
+    // fresh job tables on every (re)build: a graph that recorded this plan's launches
+    // before an autotune keeps its own tables, consistent with its recorded grids
+    p->djob_syn.renew();
+    p->djob_ana.renew();
+    for (auto &b : p->tj) b.renew();
     int rc;
     if (!p->syn.jobs.empty() && (rc = upload(p->djob_syn, p->syn.jobs.data(), p->syn.jobs.size() * sizeof(aer::Job))))
         return rc;

@@ -469,8 +469,10 @@ int aer_sht_sync(aer_sht_t cfg);
  * runs once untimed and reps times timed; synchronous; the jobs' outputs are overwritten.
  * times_ms (optional) receives the mean milliseconds per candidate (a plan: synthesis
  * candidates, then analysis candidates), choice (optional) the kept index (a plan: two).
- * aer_plan_autotune rebuilds the plan's device job tables in place: a HIP graph that
- * recorded aer_plan_run of this plan must be captured again afterwards. */
+ * aer_plan_autotune rebuilds the plan into fresh device job tables: a HIP graph that
+ * recorded aer_plan_run of this plan before keeps replaying the configuration it
+ * recorded (its tables stay allocated until aer_plan_destroy); capture again to replay
+ * the tuned one. */
 #define AER_TUNE_NCAND 4
 int aer_autotune_jobs(aer_sht_t cfg, int njob, const aer_job *jobs, int reps, double *times_ms, int *choice);
 int aer_plan_autotune(aer_plan_t plan, void *stream, int reps, double *times_ms, int *choice);

@@ -362,7 +362,10 @@ typedef struct aer_semiimp *aer_semiimp_t;
  * weight on X^(n+1). nlev <= 64. */
 int aer_semiimp_create(aer_sht_t cfg, aer_vgrid_t vg, double t_ref, double ps_ref, double dt_step,
                        double alpha, aer_semiimp_t *si);
-/* aeros_semiimp_set_step (:301-343): refactor for a new step length. */
+/* aeros_semiimp_set_step (:301-343): refactor for a new step length. The device
+ * tables are updated in place while the leapfrog takes the step length as a launch
+ * argument: a HIP graph that recorded aer_leapfrog_d replays the new tables with the
+ * step it recorded, so capture after the last set_step or set_diffusion. */
 int aer_semiimp_set_step(aer_semiimp_t si, double dt_step);
 /* Implicit del^ndiff horizontal diffusion (src/dynamics/aeros_timestep.f90:1673-1726)
  * with e-folding time tau_diff [s] at l = lmax for vorticity and temperature and
